@@ -1,0 +1,138 @@
+"""ctypes binding of libctd_hip.so, the C ABI declared in include/ctd_hip.h.
+
+The library is built in-tree (cutrialdive_amd/libctd_hip.so) by
+cutrialdive_amd/build.py.  Loading fails loudly when it is missing: there is
+no CPU fallback on the product path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libctd_hip.so")
+
+u32 = ctypes.c_uint32
+u64 = ctypes.c_uint64
+i32 = ctypes.c_int32
+
+CTD_OK = 0
+CTD_E_INVALID = -1
+CTD_E_DEVICE = -2
+CTD_E_NOMEM = -3
+CTD_E_UNSUPPORTED = -4
+CTD_E_BUSY = -5
+
+CTD_RUN_GENERIC_ONLY = 1
+
+
+class CtdJob(ctypes.Structure):
+    _fields_ = [
+        ("seq_id", u32),
+        ("max_factors", u32),
+        ("param", u64),
+        ("n0", u64),
+        ("n1", u64),
+        ("f0", u64),
+        ("f1", u64),
+    ]
+
+
+PROGRESS_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, u64, u64)
+DUE_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p)
+CHECKPOINT_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, u64)
+
+
+class CtdCallbacks(ctypes.Structure):
+    _fields_ = [
+        ("user", ctypes.c_void_p),
+        ("progress", PROGRESS_FN),
+        ("checkpoint_due", DUE_FN),
+        ("checkpoint", CHECKPOINT_FN),
+    ]
+
+
+class CtdRunOpts(ctypes.Structure):
+    _fields_ = [
+        ("batch_length", u64),
+        ("grid_size", i32),
+        ("block_size", i32),
+        ("chunk_length", u32),
+        ("flags", u32),
+    ]
+
+
+class CtdRunStats(ctypes.Structure):
+    _fields_ = [
+        ("primes", u64),
+        ("updates", u64),
+        ("tf_ms", ctypes.c_double),
+        ("sieve_ms", ctypes.c_double),
+        ("total_ms", ctypes.c_double),
+        ("tf_launches", u32),
+        ("sieve_launches", u32),
+    ]
+
+
+# every symbol include/ctd_hip.h declares, with its ctypes signature
+SIGNATURES = {
+    "ctd_hip_abi_version": (ctypes.c_int, []),
+    "ctd_hip_open": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    "ctd_hip_close": (None, [ctypes.c_void_p]),
+    "ctd_hip_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
+    "ctd_hip_device_info": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(u64), ctypes.POINTER(i32)],
+    ),
+    "ctd_hip_sieve": (
+        ctypes.c_int,
+        [ctypes.c_void_p, u64, u64, ctypes.POINTER(u64), u64, ctypes.POINTER(u64)],
+    ),
+    "ctd_hip_trial_factor": (
+        ctypes.c_int,
+        [
+            ctypes.c_void_p,
+            ctypes.POINTER(CtdJob),
+            ctypes.POINTER(CtdRunOpts),
+            ctypes.POINTER(CtdCallbacks),
+            ctypes.c_void_p,
+            ctypes.c_void_p,
+            ctypes.POINTER(u64),
+        ],
+    ),
+    "ctd_hip_trial_factor_device": (
+        ctypes.c_int,
+        [
+            ctypes.c_void_p,
+            ctypes.POINTER(CtdJob),
+            ctypes.POINTER(CtdRunOpts),
+            ctypes.c_void_p,
+            ctypes.c_void_p,
+            ctypes.POINTER(u64),
+            ctypes.c_void_p,
+        ],
+    ),
+    "ctd_hip_last_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(CtdRunStats)]),
+}
+
+_lib = None
+
+
+def load(path: str | None = None) -> ctypes.CDLL:
+    """Loads libctd_hip.so and binds every C ABI symbol; raises if it is absent."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(
+            f"{p} is missing: build the HIP engine first (python -c 'import __graft_entry__ as g; g.build()')"
+        )
+    lib = ctypes.CDLL(p)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib

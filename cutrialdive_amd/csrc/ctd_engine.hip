@@ -1,0 +1,622 @@
+// ctd_engine.hip -- MI355X trial-factoring engine behind the C ABI of ctd_hip.h.
+//
+// Replaces device_trial_factor<Seq> (trial_factoring.cuh:336-513): the segment
+// loop now runs entirely on the device.  Per batch of the prime range:
+//   1. sieve_kernel sieves 2^20-integer segments in LDS and streams the
+//      primes into HBM (no host sieve, no PCIe prime copy);
+//   2. tf kernel: one prime per lane, one index chunk per work item, the
+//      family recurrence in registers, zero residues recorded with one atomic
+//      per wave (ballot + popcount) into the factors_buffer layout.
+// Prime 2 takes the mod-2 path (trial_factoring.cuh:139-154,415-418) as an
+// index-parallel kernel.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ctd_hip.h"
+#include "ctd_families.hpp"
+#include "ctd_sieve.hpp"
+#include "ctd_tf_fast.hpp"
+
+using namespace ctd;
+
+// Generic kernel: any odd prime < 2^64, 64-bit Montgomery set-up.
+template <uint32_t FAM>
+__global__ void __launch_bounds__(256) tf_generic_kernel(JobArgs J, const uint64_t *__restrict__ primes,
+                                                          uint64_t nprimes)
+{
+    const uint64_t tr = (nprimes + 63) & ~63ull; // wave-aligned prime slots per chunk
+    const uint64_t work = tr * J.nchunks;
+    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < work;
+         w += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t c = w / tr, i = w - c * tr;
+        if (i >= nprimes) continue;
+        const uint64_t p = primes[i];
+        Mont M;
+        M.init(p);
+        const uint64_t ns = J.n0 + c * J.chunk;
+        uint64_t ne = ns + J.chunk;
+        if (ne > J.n1 || ne < ns) ne = J.n1;
+        GenState s;
+        Gen<FAM>::init(s, M, J, ns);
+        record_hits(J, ns - J.n0, p, Gen<FAM>::zero(s, M));
+        for (uint64_t n = ns; n + 1 < ne; ++n) {
+            Gen<FAM>::step(s, M, n);
+            record_hits(J, n + 1 - J.n0, p, Gen<FAM>::zero(s, M));
+        }
+    }
+}
+
+// Fast kernel: primes below 2^63 with the family's narrow step (ctd_tf_fast.hpp).
+template <uint32_t FAM>
+__global__ void __launch_bounds__(256) tf_fast_kernel(JobArgs J, const uint64_t *__restrict__ primes,
+                                                       uint64_t nprimes)
+{
+    const uint64_t tr = (nprimes + 63) & ~63ull;
+    const uint64_t work = tr * J.nchunks;
+    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < work;
+         w += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t c = w / tr, i = w - c * tr;
+        if (i >= nprimes) continue;
+        const uint64_t p = primes[i];
+        const uint64_t ns = J.n0 + c * J.chunk;
+        uint64_t ne = ns + J.chunk;
+        if (ne > J.n1 || ne < ns) ne = J.n1;
+        Fast<FAM>::run(J, p, ns, ne);
+    }
+}
+
+// Prime 2: S(n) mod 2 per index.
+__global__ void mod2_kernel(JobArgs J)
+{
+    const uint64_t N = J.n1 - J.n0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        if (value_mod_2(J.fam, J.param, J.n0 + i) == 0) {
+            uint32_t pos = atomicAdd(&J.counts[i], 1u);
+            if (pos < J.maxf) J.factors[i * J.maxf + pos] = 2;
+        }
+    }
+}
+
+__global__ void zero_u64_kernel(unsigned long long *p, uint32_t n)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+struct ctd_hip_engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::string name;
+    uint64_t total_mem = 0;
+    int cu = 0;
+    // sieving base: odd primes <= base_lim
+    uint64_t base_lim = 0;
+    uint32_t *d_sp = nullptr;
+    uint64_t *d_mu = nullptr;
+    uint32_t nsp = 0, nsmall = 0;
+    // prime stream and scratch
+    uint64_t *d_primes = nullptr;
+    uint64_t primes_cap = 0;
+    uint32_t *d_segcnt = nullptr;
+    uint64_t segcnt_cap = 0;
+    uint64_t *d_segoff = nullptr;
+    uint64_t segoff_cap = 0;
+    unsigned long long *d_scalars = nullptr; // [0] total, [1] max prime
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    ctd_run_stats stats{};
+};
+
+static std::string g_open_error;
+
+namespace {
+
+struct HipError {
+    hipError_t code;
+    const char *what;
+};
+
+#define HIPCHK(call)                                                                            \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess) throw HipError{e_, #call};                                        \
+    } while (0)
+
+int fail(ctd_hip_engine *E, int code, const std::string &msg)
+{
+    if (E) E->err = msg;
+    return code;
+}
+
+int fail_hip(ctd_hip_engine *E, const HipError &h)
+{
+    std::string m = std::string(h.what) + ": " + hipGetErrorString(h.code);
+    return fail(E, h.code == hipErrorOutOfMemory ? CTD_E_NOMEM : CTD_E_DEVICE, m);
+}
+
+template <typename T>
+void ensure_buf(T *&ptr, uint64_t &cap, uint64_t need)
+{
+    if (need <= cap && ptr) return;
+    if (ptr) HIPCHK(hipFree(ptr));
+    ptr = nullptr;
+    uint64_t n = std::max<uint64_t>(need, 1);
+    HIPCHK(hipMalloc((void **)&ptr, n * sizeof(T)));
+    cap = n;
+}
+
+uint64_t isqrt_u64(uint64_t n)
+{
+    uint64_t r = (uint64_t)std::sqrt((double)n);
+    while (r * r > n) --r;
+    while ((r + 1) * (r + 1) <= n && (r + 1) <= 0xffffffffull) ++r;
+    return r;
+}
+
+uint64_t div_up(uint64_t a, uint64_t b) { return a / b + (a % b != 0); }
+
+// Sieves [start, end) into an ordered list (u32 or u64) with the current
+// sieving numbers sp/mu.  Returns the count; allocates *out32 or *out64.
+uint64_t sieve_ordered(ctd_hip_engine *E, const uint32_t *sp, const uint64_t *mu, uint32_t nsp, uint32_t nsmall,
+                       uint64_t start, uint64_t end, uint32_t **out32, uint64_t **out64)
+{
+    if (start >= end) return 0;
+    const uint64_t lo0 = start & ~1ull;
+    const uint64_t nseg = div_up(end - lo0, SEG_SPAN);
+    ensure_buf(E->d_segcnt, E->segcnt_cap, nseg);
+    ensure_buf(E->d_segoff, E->segoff_cap, nseg);
+    SieveArgs A{};
+    A.lo0 = lo0;
+    A.start = start;
+    A.end = end;
+    A.sp = sp;
+    A.sp_mu = mu;
+    A.nsp = nsp;
+    A.nsmall = nsmall;
+    A.mode = SIEVE_COUNT;
+    A.seg_counts = E->d_segcnt;
+    A.seg_off = E->d_segoff;
+    hipLaunchKernelGGL(sieve_kernel, dim3((uint32_t)nseg), dim3(SIEVE_THREADS), 0, E->stream, A);
+    HIPCHK(hipGetLastError());
+    std::vector<uint32_t> cnt(nseg);
+    HIPCHK(hipMemcpyAsync(cnt.data(), E->d_segcnt, nseg * sizeof(uint32_t), hipMemcpyDeviceToHost, E->stream));
+    HIPCHK(hipStreamSynchronize(E->stream));
+    std::vector<uint64_t> off(nseg);
+    uint64_t total = 0;
+    for (uint64_t i = 0; i < nseg; ++i) {
+        off[i] = total;
+        total += cnt[i];
+    }
+    HIPCHK(hipMemcpyAsync(E->d_segoff, off.data(), nseg * sizeof(uint64_t), hipMemcpyHostToDevice, E->stream));
+    if (out32) {
+        HIPCHK(hipMalloc((void **)out32, std::max<uint64_t>(total, 1) * sizeof(uint32_t)));
+        A.out32 = *out32;
+        A.mode = SIEVE_ORDERED32;
+    } else {
+        HIPCHK(hipMalloc((void **)out64, std::max<uint64_t>(total, 1) * sizeof(uint64_t)));
+        A.out64 = *out64;
+        A.mode = SIEVE_ORDERED64;
+        A.cap = total;
+    }
+    hipLaunchKernelGGL(sieve_kernel, dim3((uint32_t)nseg), dim3(SIEVE_THREADS), 0, E->stream, A);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(E->stream));
+    return total;
+}
+
+uint32_t count_small(ctd_hip_engine *E, const uint32_t *d_list, uint32_t n)
+{
+    uint32_t k = std::min<uint32_t>(n, 4096);
+    std::vector<uint32_t> h(k);
+    if (k) HIPCHK(hipMemcpy(h.data(), d_list, k * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    uint32_t s = 0;
+    while (s < k && h[s] < SMALL_LIMIT) ++s;
+    return s;
+}
+
+void make_mu(ctd_hip_engine *E, const uint32_t *sp, uint64_t *mu, uint32_t n)
+{
+    if (!n) return;
+    hipLaunchKernelGGL(sieve_mu_kernel, dim3(div_up(n, 256)), dim3(256), 0, E->stream, sp, mu, n);
+    HIPCHK(hipGetLastError());
+}
+
+// Odd primes <= lim (lim <= 2^32) as the sieving base.
+void build_base(ctd_hip_engine *E, uint64_t lim)
+{
+    if (lim < 65536) lim = 65536;
+    if (E->d_sp && lim <= E->base_lim) return;
+    // level 0: odd numbers 3..255 sieve [0, 65537)
+    uint32_t *l0 = nullptr;
+    uint64_t *m0 = nullptr;
+    const uint32_t n0 = 127;
+    HIPCHK(hipMalloc((void **)&l0, n0 * sizeof(uint32_t)));
+    HIPCHK(hipMalloc((void **)&m0, n0 * sizeof(uint64_t)));
+    hipLaunchKernelGGL(odd_numbers_kernel, dim3(1), dim3(128), 0, E->stream, l0, n0);
+    make_mu(E, l0, m0, n0);
+    uint32_t *l1 = nullptr;
+    uint32_t n1 = (uint32_t)sieve_ordered(E, l0, m0, n0, n0, 3, 65537, &l1, nullptr);
+    HIPCHK(hipFree(l0));
+    HIPCHK(hipFree(m0));
+    uint64_t *m1 = nullptr;
+    HIPCHK(hipMalloc((void **)&m1, n1 * sizeof(uint64_t)));
+    make_mu(E, l1, m1, n1);
+    uint32_t s1 = count_small(E, l1, n1);
+    // level 1: primes <= 65536 sieve [3, lim + 1)
+    uint32_t *l2 = nullptr;
+    uint32_t n2 = (uint32_t)sieve_ordered(E, l1, m1, n1, s1, 3, lim + 1, &l2, nullptr);
+    HIPCHK(hipFree(l1));
+    HIPCHK(hipFree(m1));
+    uint64_t *m2 = nullptr;
+    HIPCHK(hipMalloc((void **)&m2, std::max<uint32_t>(n2, 1) * sizeof(uint64_t)));
+    make_mu(E, l2, m2, n2);
+    HIPCHK(hipStreamSynchronize(E->stream));
+    if (E->d_sp) HIPCHK(hipFree(E->d_sp));
+    if (E->d_mu) HIPCHK(hipFree(E->d_mu));
+    E->d_sp = l2;
+    E->d_mu = m2;
+    E->nsp = n2;
+    E->nsmall = count_small(E, l2, n2);
+    E->base_lim = lim;
+}
+
+bool valid_job(ctd_hip_engine *E, const ctd_job *j, int *code)
+{
+    *code = CTD_E_INVALID;
+    if (!j) return fail(E, CTD_E_INVALID, "job is NULL"), false;
+    switch (j->seq_id) {
+    case CTD_SEQ_MERSENNE: case CTD_SEQ_PROTH: case CTD_SEQ_RIESEL: case CTD_SEQ_SMARANDACHE:
+    case CTD_SEQ_FIBONACCI: case CTD_SEQ_LUCAS: case CTD_SEQ_PROTH_FIXED_N: case CTD_SEQ_RIESEL_FIXED_N:
+    case CTD_SEQ_TRIBONACCI: case CTD_SEQ_PEANO: break;
+    default: return fail(E, CTD_E_INVALID, "Unexpected number sequence id: " + std::to_string(j->seq_id)), false;
+    }
+    if (j->max_factors == 0) return fail(E, CTD_E_INVALID, "max_factors must be >= 1"), false;
+    if ((j->seq_id == CTD_SEQ_PROTH || j->seq_id == CTD_SEQ_RIESEL) && j->param == 0)
+        return fail(E, CTD_E_INVALID, std::string(j->seq_id == CTD_SEQ_PROTH ? "Proth" : "Riesel") +
+                                          " parameter k must be non-zero"), false;
+    if ((j->seq_id == CTD_SEQ_PROTH_FIXED_N || j->seq_id == CTD_SEQ_RIESEL_FIXED_N) && j->param == 0)
+        return fail(E, CTD_E_INVALID, "Invalid n value zero for Riesel/Proth fixed n"), false;
+    if (j->seq_id == CTD_SEQ_SMARANDACHE && (j->param < 2 || j->param > 64))
+        return fail(E, CTD_E_INVALID, "Base `" + std::to_string(j->param) +
+                                          "' is not supported for Smarandache numbers"), false;
+    *code = CTD_OK;
+    return true;
+}
+
+template <uint32_t FAM>
+void launch_tf(ctd_hip_engine *E, const JobArgs &J, uint64_t nprimes, bool fast, dim3 grid, dim3 block,
+               hipStream_t st)
+{
+    if (fast)
+        hipLaunchKernelGGL(tf_fast_kernel<FAM>, grid, block, 0, st, J, E->d_primes, nprimes);
+    else
+        hipLaunchKernelGGL(tf_generic_kernel<FAM>, grid, block, 0, st, J, E->d_primes, nprimes);
+}
+
+void launch_tf_any(ctd_hip_engine *E, const JobArgs &J, uint64_t nprimes, bool fast, dim3 grid, dim3 block,
+                   hipStream_t st)
+{
+    switch (J.fam) {
+    case FAM_MERSENNE: launch_tf<FAM_MERSENNE>(E, J, nprimes, fast, grid, block, st); break;
+    case FAM_PROTH: launch_tf<FAM_PROTH>(E, J, nprimes, fast, grid, block, st); break;
+    case FAM_RIESEL: launch_tf<FAM_RIESEL>(E, J, nprimes, fast, grid, block, st); break;
+    case FAM_SMARANDACHE: launch_tf<FAM_SMARANDACHE>(E, J, nprimes, fast, grid, block, st); break;
+    case FAM_FIBONACCI: launch_tf<FAM_FIBONACCI>(E, J, nprimes, fast, grid, block, st); break;
+    case FAM_LUCAS: launch_tf<FAM_LUCAS>(E, J, nprimes, fast, grid, block, st); break;
+    case FAM_PROTH_FIXED_N: launch_tf<FAM_PROTH_FIXED_N>(E, J, nprimes, fast, grid, block, st); break;
+    case FAM_RIESEL_FIXED_N: launch_tf<FAM_RIESEL_FIXED_N>(E, J, nprimes, fast, grid, block, st); break;
+    case FAM_TRIBONACCI: launch_tf<FAM_TRIBONACCI>(E, J, nprimes, fast, grid, block, st); break;
+    default: launch_tf<FAM_PEANO>(E, J, nprimes, fast, grid, block, st); break;
+    }
+}
+
+struct HostSync {
+    uint32_t *h_counts;
+    uint64_t *h_factors;
+};
+
+// The segment loop.  d_counts/d_factors hold the job's factors buffer.
+int run_job(ctd_hip_engine *E, const ctd_job *job, const ctd_run_opts *opts, const ctd_callbacks *cb,
+            uint32_t *d_counts, uint64_t *d_factors, uint64_t *last_prime, hipStream_t st, const HostSync *hs)
+{
+    int code;
+    if (!valid_job(E, job, &code)) return code;
+    E->stats = ctd_run_stats{};
+    if (last_prime) *last_prime = 0;
+    if (job->n1 <= job->n0 || job->f1 <= job->f0) return CTD_OK; // trial_factoring_impl.hpp:380-387
+    const uint64_t N = job->n1 - job->n0;
+    JobArgs J{};
+    J.fam = job->seq_id;
+    J.maxf = job->max_factors;
+    J.param = job->param;
+    J.n0 = job->n0;
+    J.n1 = job->n1;
+    J.counts = d_counts;
+    J.factors = d_factors;
+
+    hipEvent_t t_all0, t_all1;
+    HIPCHK(hipEventCreate(&t_all0));
+    HIPCHK(hipEventCreate(&t_all1));
+    HIPCHK(hipEventRecord(t_all0, st));
+
+    uint64_t f0 = job->f0;
+    const uint64_t f1 = job->f1;
+    if (f0 <= 2 && f1 > 2) {
+        J.chunk = N;
+        J.nchunks = 1;
+        uint32_t g = (uint32_t)std::min<uint64_t>(div_up(N, 256), 4096);
+        hipLaunchKernelGGL(mod2_kernel, dim3(g), dim3(256), 0, st, J);
+        HIPCHK(hipGetLastError());
+        f0 = 3;
+        if (last_prime) *last_prime = 2;
+    }
+    uint64_t lastp = (job->f0 <= 2 && f1 > 2) ? 2 : 0;
+    if (f0 < f1) {
+        build_base(E, isqrt_u64(f1 - 1));
+        const bool generic_only = opts && (opts->flags & CTD_RUN_GENERIC_ONLY);
+        const bool fast = !generic_only && Fast<0>::fits(f1 - 1);
+        uint64_t blen = (opts && opts->batch_length) ? opts->batch_length : (1ull << 31);
+        blen = std::max<uint64_t>(div_up(blen, SEG_SPAN), 1) * SEG_SPAN;
+        const uint64_t lo0 = f0 & ~1ull;
+        for (uint64_t blo = lo0; blo < f1;) {
+            uint64_t bhi = (f1 - blo > blen) ? blo + blen : f1;
+            uint64_t bstart = std::max(blo, f0);
+            uint64_t len = bhi - blo;
+            uint64_t nseg = div_up(len, SEG_SPAN);
+            // Montgomery-Vaughan: pi(x+y) - pi(x) <= 2y / log y
+            double ly = std::log((double)std::max<uint64_t>(len, 16));
+            uint64_t cap = (uint64_t)(2.0 * (double)len / ly) + 4096;
+            ensure_buf(E->d_primes, E->primes_cap, cap);
+            ensure_buf(E->d_segcnt, E->segcnt_cap, nseg);
+            if (!E->d_scalars) HIPCHK(hipMalloc((void **)&E->d_scalars, 2 * sizeof(unsigned long long)));
+            hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, st, E->d_scalars, 2u);
+            SieveArgs A{};
+            A.lo0 = blo;
+            A.start = bstart;
+            A.end = bhi;
+            A.sp = E->d_sp;
+            A.sp_mu = E->d_mu;
+            A.nsp = E->nsp;
+            A.nsmall = E->nsmall;
+            A.mode = SIEVE_DENSE;
+            A.seg_counts = E->d_segcnt;
+            A.out64 = E->d_primes;
+            A.total = E->d_scalars;
+            A.maxp = E->d_scalars + 1;
+            A.cap = E->primes_cap;
+            HIPCHK(hipEventRecord(E->ev0, st));
+            hipLaunchKernelGGL(sieve_kernel, dim3((uint32_t)nseg), dim3(SIEVE_THREADS), 0, st, A);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipEventRecord(E->ev1, st));
+            unsigned long long sc[2];
+            HIPCHK(hipMemcpyAsync(sc, E->d_scalars, sizeof sc, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, E->ev0, E->ev1));
+            E->stats.sieve_ms += ms;
+            E->stats.sieve_launches++;
+            uint64_t np = sc[0];
+            if (np > E->primes_cap)
+                return fail(E, CTD_E_DEVICE, "prime buffer overflow in batch starting at " + std::to_string(blo));
+            if (np) {
+                // index chunking so that small prime batches still fill the device
+                uint64_t chunk = N, nch = 1;
+                if (opts && opts->chunk_length) {
+                    chunk = opts->chunk_length;
+                    nch = div_up(N, chunk);
+                } else {
+                    const uint64_t target = (uint64_t)E->cu * 4096;
+                    if (np < target && N > 2048) {
+                        nch = std::min<uint64_t>(div_up(target, np), div_up(N, 1024));
+                        chunk = div_up(N, nch);
+                        nch = div_up(N, chunk);
+                    }
+                }
+                J.chunk = chunk;
+                J.nchunks = (uint32_t)nch;
+                uint32_t block = (opts && opts->block_size > 0) ? (uint32_t)opts->block_size : 256u;
+                block = std::max<uint32_t>(64, std::min<uint32_t>(block, 256)) & ~63u;
+                uint64_t items = ((np + 63) & ~63ull) * nch;
+                uint64_t grid = (opts && opts->grid_size > 0) ? (uint64_t)opts->grid_size : div_up(items, block);
+                grid = std::min<uint64_t>(grid, 1ull << 30);
+                HIPCHK(hipEventRecord(E->ev0, st));
+                launch_tf_any(E, J, np, fast, dim3((uint32_t)grid), dim3(block), st);
+                HIPCHK(hipGetLastError());
+                HIPCHK(hipEventRecord(E->ev1, st));
+                HIPCHK(hipStreamSynchronize(st));
+                HIPCHK(hipEventElapsedTime(&ms, E->ev0, E->ev1));
+                E->stats.tf_ms += ms;
+                E->stats.tf_launches++;
+                E->stats.primes += np;
+                E->stats.updates += np * N;
+                lastp = sc[1];
+            }
+            if (last_prime) *last_prime = lastp;
+            if (cb && cb->progress) cb->progress(cb->user, bhi, lastp);
+            if (cb && cb->checkpoint && hs && cb->checkpoint_due && cb->checkpoint_due(cb->user)) {
+                HIPCHK(hipMemcpyAsync(hs->h_counts, d_counts, N * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+                HIPCHK(hipMemcpyAsync(hs->h_factors, d_factors, N * (uint64_t)job->max_factors * sizeof(uint64_t),
+                                      hipMemcpyDeviceToHost, st));
+                HIPCHK(hipStreamSynchronize(st));
+                cb->checkpoint(cb->user, lastp);
+            }
+            blo = bhi;
+        }
+    }
+    HIPCHK(hipEventRecord(t_all1, st));
+    HIPCHK(hipStreamSynchronize(st));
+    float tot = 0;
+    HIPCHK(hipEventElapsedTime(&tot, t_all0, t_all1));
+    E->stats.total_ms = tot;
+    HIPCHK(hipEventDestroy(t_all0));
+    HIPCHK(hipEventDestroy(t_all1));
+    if (last_prime) *last_prime = lastp;
+    return CTD_OK;
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int ctd_hip_abi_version(void) { return CTD_HIP_ABI_VERSION; }
+
+int ctd_hip_open(int device, ctd_hip_engine **out)
+{
+    if (!out) return CTD_E_INVALID;
+    *out = nullptr;
+    ctd_hip_engine *E = new ctd_hip_engine;
+    try {
+        int n = 0;
+        HIPCHK(hipGetDeviceCount(&n));
+        if (device < 0 || device >= n) {
+            g_open_error = "Error selecting device " + std::to_string(device) + ": no such device";
+            delete E;
+            return CTD_E_INVALID;
+        }
+        E->device = device;
+        HIPCHK(hipSetDevice(device));
+        hipDeviceProp_t prop;
+        HIPCHK(hipGetDeviceProperties(&prop, device));
+        E->name = prop.name;
+        E->total_mem = prop.totalGlobalMem;
+        E->cu = prop.multiProcessorCount;
+        HIPCHK(hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking));
+        HIPCHK(hipEventCreate(&E->ev0));
+        HIPCHK(hipEventCreate(&E->ev1));
+    } catch (const HipError &h) {
+        g_open_error = std::string(h.what) + ": " + hipGetErrorString(h.code);
+        delete E;
+        return CTD_E_DEVICE;
+    }
+    *out = E;
+    return CTD_OK;
+}
+
+void ctd_hip_close(ctd_hip_engine *E)
+{
+    if (!E) return;
+    (void)hipSetDevice(E->device);
+    if (E->stream) (void)hipStreamSynchronize(E->stream);
+    (void)hipFree(E->d_sp);
+    (void)hipFree(E->d_mu);
+    (void)hipFree(E->d_primes);
+    (void)hipFree(E->d_segcnt);
+    (void)hipFree(E->d_segoff);
+    (void)hipFree(E->d_scalars);
+    if (E->ev0) (void)hipEventDestroy(E->ev0);
+    if (E->ev1) (void)hipEventDestroy(E->ev1);
+    if (E->stream) (void)hipStreamDestroy(E->stream);
+    delete E;
+}
+
+const char *ctd_hip_last_error(const ctd_hip_engine *E) { return E ? E->err.c_str() : g_open_error.c_str(); }
+
+int ctd_hip_device_info(ctd_hip_engine *E, char *name, size_t name_len, uint64_t *total_mem, int32_t *cu_count)
+{
+    if (!E) return CTD_E_INVALID;
+    if (name && name_len) {
+        std::snprintf(name, name_len, "%s", E->name.c_str());
+    }
+    if (total_mem) *total_mem = E->total_mem;
+    if (cu_count) *cu_count = E->cu;
+    return CTD_OK;
+}
+
+int ctd_hip_last_stats(const ctd_hip_engine *E, ctd_run_stats *out)
+{
+    if (!E || !out) return CTD_E_INVALID;
+    *out = E->stats;
+    return CTD_OK;
+}
+
+int ctd_hip_sieve(ctd_hip_engine *E, uint64_t start, uint64_t end, uint64_t *out, uint64_t cap, uint64_t *count)
+{
+    if (!E || !count) return fail(E, CTD_E_INVALID, "invalid argument");
+    *count = 0;
+    if (start >= end) return CTD_OK;
+    try {
+        HIPCHK(hipSetDevice(E->device));
+        uint64_t n2 = (start <= 2 && end > 2) ? 1 : 0;
+        uint64_t s = std::max<uint64_t>(start, 3);
+        uint64_t n = 0;
+        uint64_t *d = nullptr;
+        if (s < end) {
+            build_base(E, isqrt_u64(end - 1));
+            n = sieve_ordered(E, E->d_sp, E->d_mu, E->nsp, E->nsmall, s, end, nullptr, &d);
+        }
+        *count = n + n2;
+        if (out && cap) {
+            if (n2) out[0] = 2;
+            uint64_t k = std::min<uint64_t>(n, cap - n2 > cap ? 0 : cap - n2);
+            if (k) HIPCHK(hipMemcpy(out + n2, d, k * sizeof(uint64_t), hipMemcpyDeviceToHost));
+        }
+        if (d) HIPCHK(hipFree(d));
+    } catch (const HipError &h) {
+        return fail_hip(E, h);
+    }
+    return CTD_OK;
+}
+
+int ctd_hip_trial_factor_device(ctd_hip_engine *E, const ctd_job *job, const ctd_run_opts *opts,
+                                uint32_t *d_counts, uint64_t *d_factors, uint64_t *last_prime, void *stream)
+{
+    if (!E) return CTD_E_INVALID;
+    try {
+        HIPCHK(hipSetDevice(E->device));
+        hipStream_t st = stream ? (hipStream_t)stream : E->stream;
+        return run_job(E, job, opts, nullptr, d_counts, d_factors, last_prime, st, nullptr);
+    } catch (const HipError &h) {
+        return fail_hip(E, h);
+    }
+}
+
+int ctd_hip_trial_factor(ctd_hip_engine *E, const ctd_job *job, const ctd_run_opts *opts, const ctd_callbacks *cb,
+                         uint32_t *counts, uint64_t *factors, uint64_t *last_prime)
+{
+    if (!E) return CTD_E_INVALID;
+    int code;
+    if (!valid_job(E, job, &code)) return code;
+    if (!counts || !factors) return fail(E, CTD_E_INVALID, "counts/factors are NULL");
+    uint32_t *dc = nullptr;
+    uint64_t *df = nullptr;
+    try {
+        HIPCHK(hipSetDevice(E->device));
+        if (job->n1 <= job->n0) return CTD_OK;
+        const uint64_t N = job->n1 - job->n0;
+        const uint64_t nf = N * (uint64_t)job->max_factors;
+        HIPCHK(hipMalloc((void **)&dc, N * sizeof(uint32_t)));
+        HIPCHK(hipMalloc((void **)&df, nf * sizeof(uint64_t)));
+        HIPCHK(hipMemcpyAsync(dc, counts, N * sizeof(uint32_t), hipMemcpyHostToDevice, E->stream));
+        HIPCHK(hipMemcpyAsync(df, factors, nf * sizeof(uint64_t), hipMemcpyHostToDevice, E->stream));
+        HostSync hs{counts, factors};
+        int rc = run_job(E, job, opts, cb, dc, df, last_prime, E->stream, &hs);
+        if (rc == CTD_OK) {
+            HIPCHK(hipMemcpyAsync(counts, dc, N * sizeof(uint32_t), hipMemcpyDeviceToHost, E->stream));
+            HIPCHK(hipMemcpyAsync(factors, df, nf * sizeof(uint64_t), hipMemcpyDeviceToHost, E->stream));
+            HIPCHK(hipStreamSynchronize(E->stream));
+        }
+        HIPCHK(hipFree(dc));
+        HIPCHK(hipFree(df));
+        return rc;
+    } catch (const HipError &h) {
+        if (dc) (void)hipFree(dc);
+        if (df) (void)hipFree(df);
+        return fail_hip(E, h);
+    }
+}
+
+} // extern "C"

@@ -1,0 +1,145 @@
+/*
+ * ctd_hip.h -- C ABI of the MI355X (gfx950) trial-factoring engine.
+ *
+ * This is the thin boundary the C++ host (CLI, number_sequence plugins,
+ * checkpointing) calls into.  Signatures hold only plain integers, pointers
+ * and sizes.  Every entry point returns CTD_OK (0) or a negative CTD_E_* code;
+ * ctd_hip_last_error() then describes the failure.
+ *
+ * Each declaration names the reference interface it replaces (paths are under
+ * the reference's src/).
+ */
+#ifndef CTD_HIP_H
+#define CTD_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CTD_HIP_ABI_VERSION 1
+
+enum {
+    CTD_OK = 0,
+    CTD_E_INVALID = -1,   /* bad argument (host validation) */
+    CTD_E_DEVICE = -2,    /* HIP runtime / kernel error */
+    CTD_E_NOMEM = -3,     /* device allocation failed */
+    CTD_E_UNSUPPORTED = -4,
+    CTD_E_BUSY = -5       /* serving queue full */
+};
+
+/* Sequence ids.  The values are the persisted order of num_seq_id
+ * (num_seq_id.hpp:16-27); CTD_SEQ_PEANO is a test-only sequence S(n) = n
+ * (tests/peano.hpp:127-137) and is never persisted. */
+enum {
+    CTD_SEQ_MERSENNE = 0,
+    CTD_SEQ_PROTH = 1,
+    CTD_SEQ_RIESEL = 2,
+    CTD_SEQ_SMARANDACHE = 3,
+    CTD_SEQ_FIBONACCI = 4,
+    CTD_SEQ_LUCAS = 5,
+    CTD_SEQ_PROTH_FIXED_N = 6,
+    CTD_SEQ_RIESEL_FIXED_N = 7,
+    CTD_SEQ_TRIBONACCI = 8,
+    CTD_SEQ_PEANO = 255
+};
+
+/* One trial-factoring job: the fields of trial_factoring_options
+ * (trial_factoring_options.hpp:25-39) plus the parsed sequence parameter of
+ * num_seq_spec (num_seq_dispatch.hpp:266-315). */
+typedef struct ctd_job {
+    uint32_t seq_id;        /* CTD_SEQ_* */
+    uint32_t max_factors;   /* max_factors_per_number (>= 1) */
+    uint64_t param;         /* k (proth, riesel), n (fixed-n), base (smarandache, 2..64) */
+    uint64_t n0, n1;        /* index range [n0, n1) */
+    uint64_t f0, f1;        /* prime range [f0, f1); f0 already moved past a resumed prime */
+} ctd_job;
+
+/* Host callbacks around the segment loop of device_trial_factor
+ * (trial_factoring.cuh:437-488).  Any member may be NULL. */
+typedef struct ctd_callbacks {
+    void *user;
+    /* After each completed prime batch: upper bound of the batch and the last
+     * prime processed (progress::update, progress.cpp:70-104). */
+    void (*progress)(void *user, uint64_t bound, uint64_t last_prime);
+    /* Nonzero when a checkpoint is due (checkpoint_manager::due, checkpoint.cpp:166-170). */
+    int (*checkpoint_due)(void *user);
+    /* The host counts/factors arrays were refreshed and hold every factor of
+     * every prime <= last_prime (trial_factoring.cuh:455-465). */
+    void (*checkpoint)(void *user, uint64_t last_prime);
+} ctd_callbacks;
+
+/* Launch shape overrides: --grid-size / --block-size (command_line.cpp:148-164)
+ * and --segment-length (trial_factoring_options.cpp:597).  Zero = autotuned. */
+typedef struct ctd_run_opts {
+    uint64_t batch_length;  /* integers of the prime range per device batch */
+    int32_t grid_size;
+    int32_t block_size;
+    uint32_t chunk_length;  /* indices per work item (index chunking) */
+    uint32_t flags;         /* CTD_RUN_* */
+} ctd_run_opts;
+
+#define CTD_RUN_GENERIC_ONLY 1u /* force the generic 64-bit Montgomery kernels */
+
+typedef struct ctd_hip_engine ctd_hip_engine;
+
+/* Opens the engine on a device (cudaSetDevice + device properties,
+ * trial_factoring.cuh:349-363). */
+int ctd_hip_open(int device, ctd_hip_engine **out);
+/* Releases every device resource of the engine. */
+void ctd_hip_close(ctd_hip_engine *eng);
+/* Message of the last failure on this engine (or of the last failed open when
+ * eng is NULL).  The string stays valid until the next call on the engine. */
+const char *ctd_hip_last_error(const ctd_hip_engine *eng);
+/* Device name, total memory and compute-unit count
+ * (print_gpu_device_info, trial_factoring.cuh:241-252). */
+int ctd_hip_device_info(ctd_hip_engine *eng, char *name, size_t name_len,
+                        uint64_t *total_mem, int32_t *cu_count);
+/* ABI version of the loaded library (CTD_HIP_ABI_VERSION). */
+int ctd_hip_abi_version(void);
+
+/* Prime supply: primes of [start, end) in ascending order, generated by the
+ * device sieve.  Replaces siever::sieve (siever.hpp:179-237) plus the H2D
+ * copy of device_prime_data::copy_from_host (prime_data.hpp:165-196).
+ * Writes at most cap primes to out (host memory) and the total to *count. */
+int ctd_hip_sieve(ctd_hip_engine *eng, uint64_t start, uint64_t end,
+                  uint64_t *out, uint64_t cap, uint64_t *count);
+
+/* Plain single-job form: replaces device_trial_factor<Seq>
+ * (trial_factoring.cuh:336-513) for the built-in families.
+ * counts[n1-n0] and factors[(n1-n0)*max_factors] are host arrays in the
+ * factors_buffer layout (factors_buffer.hpp:77-86); they are read (resume
+ * state) and written back.  *last_prime receives the last prime processed
+ * (0 if none).  opts and cb may be NULL. */
+int ctd_hip_trial_factor(ctd_hip_engine *eng, const ctd_job *job,
+                         const ctd_run_opts *opts, const ctd_callbacks *cb,
+                         uint32_t *counts, uint64_t *factors, uint64_t *last_prime);
+
+/* Same as ctd_hip_trial_factor with counts/factors already in device memory
+ * (d_counts, d_factors) and no host copies: the form timed with its buffers
+ * resident in HBM.  stream is a hipStream_t (NULL = the engine's stream). */
+int ctd_hip_trial_factor_device(ctd_hip_engine *eng, const ctd_job *job,
+                                const ctd_run_opts *opts, uint32_t *d_counts,
+                                uint64_t *d_factors, uint64_t *last_prime,
+                                void *stream);
+
+/* Statistics of the last run on this engine: primes tested (odd primes of the
+ * window), residue updates (primes x indices), kernel milliseconds of the
+ * trial-division kernels and of the sieve kernels (HIP events). */
+typedef struct ctd_run_stats {
+    uint64_t primes;
+    uint64_t updates;
+    double tf_ms;
+    double sieve_ms;
+    double total_ms;
+    uint32_t tf_launches;
+    uint32_t sieve_launches;
+} ctd_run_stats;
+int ctd_hip_last_stats(const ctd_hip_engine *eng, ctd_run_stats *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,192 @@
+"""GPU parity: the HIP engine (through the C ABI) against the oracle.
+
+Factor lists must be bit-identical: same primes per index, same counts.
+Sizes are chosen so the oracle finishes in seconds.
+"""
+import numpy as np
+import pytest
+
+import cutrialdive_amd as ctd
+from cutrialdive_amd import _native as nat
+from oracle import oracle as o
+
+pytestmark = pytest.mark.gpu
+
+FAMILIES = [
+    (o.MERSENNE, 0),
+    (o.PROTH, 1279),
+    (o.RIESEL, 37),
+    (o.PROTH, 3),
+    (o.RIESEL, 78557),
+    (o.SMARANDACHE, 10),
+    (o.SMARANDACHE, 16),
+    (o.SMARANDACHE, 3),
+    (o.SMARANDACHE, 2),
+    (o.SMARANDACHE, 64),
+    (o.FIBONACCI, 0),
+    (o.LUCAS, 0),
+    (o.PROTH_FIXED_N, 1000),
+    (o.RIESEL_FIXED_N, 333),
+    (o.TRIBONACCI, 0),
+    (o.PEANO, 0),
+]
+
+
+def gpu_lists(eng, seq, param, n0, n1, f0, f1, mx=1024, **kw):
+    opts = None
+    if kw:
+        opts = nat.CtdRunOpts(kw.get("batch_length", 0), 0, 0, kw.get("chunk_length", 0), kw.get("flags", 0))
+    counts, factors, last = eng.run_job(seq, param, n0, n1, f0, f1, mx, opts=opts)
+    res = ctd.FactoringResults.from_buffer(n0, counts, factors, mx)
+    return res.factors, counts, last
+
+
+def oracle_counts(seq, param, n0, n1, f0, f1, mx=1024):
+    c, _ = o.trial_factor_raw(seq, param, n0, n1, f0, f1, mx)
+    return c
+
+
+def assert_same(g, gc, seq, param, n0, n1, f0, f1, mx=1024):
+    """Counts equal everywhere; stored lists equal wherever they did not
+    overflow max_factors (beyond it the reference stores an arbitrary subset,
+    factors_buffer.hpp:110-121)."""
+    rc, rf = o.trial_factor_raw(seq, param, n0, n1, f0, f1, mx)
+    assert np.array_equal(gc, rc)
+    r = [sorted(int(x) for x in rf[i * mx : i * mx + min(int(c), mx)]) for i, c in enumerate(rc)]
+    for i, c in enumerate(rc):
+        if c <= mx:
+            assert g[i] == r[i], (n0 + i, g[i], r[i])
+
+
+@pytest.mark.parametrize(
+    "start,end",
+    [
+        (256, 307),
+        (1 << 16, (1 << 16) + 100),
+        ((1 << 31) - 1, (1 << 31) + 100),
+        (3299100100, 3299100100 + 256),
+        (1, 1 << 17),
+        ((1 << 32) - 256, (1 << 32) + 256),
+        (65537 * 65537 - 200, 65537 * 65537 + 1),
+        ((1 << 34) - 200, (1 << 34) + 200),
+        ((1 << 41) - (1 << 37) + (1 << 32) - 1, (1 << 41) - (1 << 37) + (1 << 32) + 1000),
+        (0, 3),
+        (2, 3),
+        (1 << 40, (1 << 40) + (1 << 22) + 12345),
+    ],
+)
+def test_sieve_windows(engine, start, end):
+    g = engine.sieve(start, end)
+    r = o.sieve(start, end)
+    assert np.array_equal(g, r)
+
+
+def test_sieve_pi_2_30(engine):
+    assert len(engine.sieve(0, 1 << 30)) == 54400028
+
+
+@pytest.mark.parametrize("seq,param", FAMILIES)
+def test_family_small(engine, seq, param):
+    n0, n1 = (1, 300) if seq != o.PROTH_FIXED_N and seq != o.RIESEL_FIXED_N else (3, 400)
+    f0, f1 = 0, 1 << 20
+    g, gc, _ = gpu_lists(engine, seq, param, n0, n1, f0, f1)
+    # tribonacci T(1) = 0: index 0 overflows max_factors
+    assert_same(g, gc, seq, param, n0, n1, f0, f1)
+
+
+@pytest.mark.parametrize("seq,param", FAMILIES)
+def test_family_generic_kernel(engine, seq, param):
+    n0, n1 = 50, 250
+    f0, f1 = 1 << 20, (1 << 20) + (1 << 18)
+    g, _, _ = gpu_lists(engine, seq, param, n0, n1, f0, f1, flags=nat.CTD_RUN_GENERIC_ONLY)
+    r = o.trial_factor(seq, param, n0, n1, f0, f1)
+    assert g == r
+
+
+@pytest.mark.parametrize("seq,param", FAMILIES)
+def test_family_w64_window(engine, seq, param):
+    # primes above 2^32: the 64-bit step paths
+    n0, n1 = 100, 400
+    f0, f1 = 1 << 36, (1 << 36) + (1 << 16)
+    g, _, _ = gpu_lists(engine, seq, param, n0, n1, f0, f1)
+    r = o.trial_factor(seq, param, n0, n1, f0, f1)
+    assert g == r
+
+
+@pytest.mark.parametrize("seq,param", FAMILIES)
+def test_family_chunked(engine, seq, param):
+    # index chunks start mid-range: jump-ahead must equal propagation
+    n0, n1 = 7, 3000
+    f0, f1 = 1000, 20000
+    g, _, _ = gpu_lists(engine, seq, param, n0, n1, f0, f1, chunk_length=97)
+    r = o.trial_factor(seq, param, n0, n1, f0, f1)
+    assert g == r
+
+
+def test_reference_tf_cases(engine):
+    # tests/tf_tests.cpp:70-109
+    g, _, _ = gpu_lists(engine, o.MERSENNE, 0, 1, 101, 0, 1 << 16)
+    assert g[96] == [11447]
+    g, _, _ = gpu_lists(engine, o.PROTH, 1279, 0, 80, 0, 1 << 16, 128)
+    assert g[79] == [3, 2063]
+    # tests/tf_tests.cpp:18-67 (S(n) = n)
+    g, _, _ = gpu_lists(engine, o.PEANO, 0, 2956708207, 2956708707, 0, 1 << 24)
+    assert g[0] == []
+    assert g[1] == [2, 43, 4297541]
+    assert g[2] == [3, 7, 13, 107, 127, 797]
+    assert g[-1] == [2, 7, 4310071]
+
+
+def test_readme_smarandache_lines(engine):
+    # README.md:64-70, Sm10 to 2^37: checked on windows around each listed factor
+    expect = {10: [2, 5, 1234567891], 11: [3, 7, 13, 67, 107, 630803], 12: [2, 3, 2437, 2110805449]}
+    g, _, _ = gpu_lists(engine, o.SMARANDACHE, 10, 10, 13, 0, 1 << 32)
+    for n, fs in expect.items():
+        assert g[n - 10] == fs
+
+
+def test_cfg1_reduced(engine):
+    # Mersenne n in [2, 4000), p < 2^24 (config 1 at reduced size)
+    g, gc, _ = gpu_lists(engine, o.MERSENNE, 0, 2, 4000, 0, 1 << 24)
+    r = o.trial_factor(o.MERSENNE, 0, 2, 4000, 0, 1 << 24)
+    assert g == r
+
+
+def test_cfg2_subwindow(engine):
+    # Sm10 n in [10, 1000) over [2^32, 2^32 + 2^22) (config 2 sub-window)
+    f0, f1 = 1 << 32, (1 << 32) + (1 << 22)
+    g, _, _ = gpu_lists(engine, o.SMARANDACHE, 10, 10, 1000, f0, f1)
+    r = o.trial_factor(o.SMARANDACHE, 10, 10, 1000, f0, f1)
+    assert g == r
+
+
+def test_cfg3_subwindow(engine):
+    # Sm16 n in [1, 2000) and Proth k=3 n in [1, 20000) over [2^40, 2^40 + 2^20)
+    f0, f1 = 1 << 40, (1 << 40) + (1 << 20)
+    g, _, _ = gpu_lists(engine, o.SMARANDACHE, 16, 1, 2000, f0, f1)
+    assert g == o.trial_factor(o.SMARANDACHE, 16, 1, 2000, f0, f1)
+    g, _, _ = gpu_lists(engine, o.PROTH, 3, 1, 20000, f0, f1)
+    assert g == o.trial_factor(o.PROTH, 3, 1, 20000, f0, f1)
+
+
+def test_n0_zero_quirks(engine):
+    # Fibonacci / Lucas starting at n0 = 0 (fibonacci.hpp:68-71): every prime
+    # divides every index, so the counts overflow max_factors identically
+    for seq in (o.FIBONACCI, o.LUCAS, o.TRIBONACCI, o.MERSENNE, o.SMARANDACHE):
+        param = 10 if seq == o.SMARANDACHE else 0
+        _, gc, _ = gpu_lists(engine, seq, param, 0, 40, 0, 5000, mx=8)
+        rc = oracle_counts(seq, param, 0, 40, 0, 5000, mx=8)
+        assert np.array_equal(gc, rc), seq
+
+
+def test_small_batches_and_resume_accumulate(engine):
+    # several prime batches and a two-part run equal one run
+    seq, param, n0, n1 = o.PROTH, 1279, 0, 500
+    mx = 256
+    full, fc, last = gpu_lists(engine, seq, param, n0, n1, 0, 1 << 22, mx=mx, batch_length=1 << 20)
+    c1, f1_, last1 = engine.run_job(seq, param, n0, n1, 0, 1 << 21, mx)
+    c2, f2_, last2 = engine.run_job(seq, param, n0, n1, last1 + 1, 1 << 22, mx, counts=c1, factors=f1_)
+    res = ctd.FactoringResults.from_buffer(n0, c2, f2_, mx)
+    assert res.factors == full
+    assert last2 == last
+    assert last1 == int(o.sieve((1 << 21) - 100, 1 << 21)[-1])
