@@ -1,0 +1,204 @@
+#!/usr/bin/env python3
+"""Prime x index residue updates/s of the MI355X trial-factoring engine.
+
+Default workload (config 2 of the baseline): Smarandache base 10,
+n in [10, 1000), every prime in [2^32, 2^36), plain single-job form.
+One step is one device batch: 2^31 consecutive integers of the prime window
+sieved on the device and every prime of it propagated over all 990 indices.
+The default 30 timed steps cover the whole window [2^32, 2^36) once.
+
+Prints one JSON line (see DESIGN.md for the field definitions).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+WORKLOADS = {
+    # name: (seq_id, param, n0, n1, F0, F1, batch, description)
+    "cfg2": (3, 10, 10, 1000, 1 << 32, 1 << 36, 1 << 31,
+             "smarandache,10 n in [10,1000), primes in [2^32,2^36), plain single-job form"),
+    "cfg1": (0, 0, 2, 20000, 0, 1 << 30, 1 << 27,
+             "mersenne n in [2,20000), primes < 2^30, plain single-job form"),
+    "cfg3a": (3, 16, 1, 2000, 1 << 40, (1 << 40) + (1 << 36), 1 << 31,
+              "smarandache,16 n in [1,2000), primes in [2^40, 2^40+2^36) (sub-window of [2^40,2^42))"),
+    "cfg3b": (1, 3, 1, 100000, 1 << 40, (1 << 40) + (1 << 32), 1 << 28,
+              "proth,3 n in [1,100000), primes in [2^40, 2^40+2^32) (sub-window of [2^40,2^42))"),
+}
+FAMILY_NAMES = {0: "mersenne", 1: "proth", 2: "riesel", 3: "smarandache", 4: "fibonacci", 5: "lucas",
+                6: "proth_fixed_n", 7: "riesel_fixed_n", 8: "tribonacci"}
+METRIC = "prime×index residue updates/s, batched vs plain on same GPU, bit-exact factors"
+
+
+def ops_per_update(seq: int, param: int, pmax: int) -> float:
+    with open(os.path.join(ROOT, "cutrialdive_amd", "ops_per_update.json")) as f:
+        t = json.load(f)
+    fam = t[FAMILY_NAMES[seq]]
+    if seq == 3 and (param & (param - 1)) == 0:
+        return float(fam.get("w64_pow2_base", fam["w64"]))
+    return float(fam["w32"] if pmax < (1 << 31) else fam["w64"])
+
+
+def cpu_baseline(seq, param, n0, n1, f0, sample_len, threads):
+    from oracle import oracle as o
+
+    f1 = f0 + sample_len
+    t0 = time.perf_counter()
+    counts, factors = o.trial_factor_raw(seq, param, n0, n1, f0, f1, 1024, threads)
+    dt = time.perf_counter() - t0
+    primes = o.job_primes(seq, param, n0, n1, f0, f1)
+    return primes * (n1 - n0) / dt, dt, primes, counts, factors
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="cfg2", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-sample-bits", type=int, default=29)
+    ap.add_argument("--max-factors", type=int, default=1024)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (--gpus {args.gpus} requested); the engine shards by prime "
+              "window across GPUs with no exchange in the hot loop, and multi-GPU runs are not part of this build.",
+              file=sys.stderr)
+        return 2
+
+    import torch
+
+    import cutrialdive_amd as ctd
+    from cutrialdive_amd import _native as nat
+
+    seq, param, n0, n1, F0, F1, batch, desc = WORKLOADS[args.workload]
+    N = n1 - n0
+    mx = args.max_factors
+    nbatches = (F1 - F0 + batch - 1) // batch
+    eng = ctd.Engine(0)
+    name, mem, cu = eng.device_info()
+    print(f"# device {name}, {cu} CUs, {mem / 2**30:.0f} GiB; workload {args.workload}: {desc}", file=sys.stderr,
+          flush=True)
+
+    d_counts = torch.zeros(N, dtype=torch.int32, device="cuda")
+    d_factors = torch.zeros(N * mx, dtype=torch.int64, device="cuda")
+    torch.cuda.synchronize()
+
+    def run_batch(k: int):
+        lo = F0 + (k % nbatches) * batch
+        hi = min(lo + batch, F1)
+        job = nat.CtdJob(seq, mx, param, n0, n1, lo, hi)
+        eng.run_job_device(job, d_counts.data_ptr(), d_factors.data_ptr())
+        return eng.last_stats()
+
+    for k in range(args.warmup):
+        run_batch(k)
+    d_counts.zero_()
+    d_factors.zero_()
+    torch.cuda.synchronize()
+
+    updates = primes = 0
+    tf_ms = sieve_ms = 0.0
+    launches = 0
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        st = run_batch(k)
+        updates += st.updates
+        primes += st.primes
+        tf_ms += st.tf_ms
+        sieve_ms += st.sieve_ms
+        launches += st.tf_launches
+        print(f"# step {k}: {st.primes} primes, tf {st.tf_ms:.1f} ms, sieve {st.sieve_ms:.1f} ms", file=sys.stderr,
+              flush=True)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+
+    value = updates / dt
+    c_upd = ops_per_update(seq, param, F1 - 1)
+    peak = 39.3  # T int32 lane-ops/s
+    avg_launch_s = tf_ms / 1e3 / max(launches, 1)
+    per_launch_ops = updates / max(launches, 1) * c_upd
+    achieved = per_launch_ops / avg_launch_s / 1e12
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {
+            "workload": desc,
+            "sequence": FAMILY_NAMES[seq],
+            "param": param,
+            "n0": n0,
+            "n1": n1,
+            "prime_window": [F0, F1],
+            "batch_integers": batch,
+            "primes_timed": primes,
+            "updates_timed": updates,
+            "form": "plain",
+        },
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved,
+            "peak": peak,
+            "unit": "T int32 lane-ops/s",
+            "frac": achieved / peak,
+            "traffic": None,
+            "kernel": f"tf_fast_kernel<{FAMILY_NAMES[seq]}>",
+            "ops_per_update": c_upd,
+            "avg_launch_ms": avg_launch_s * 1e3,
+        },
+        "sieve": {
+            "primes_per_s": primes / (sieve_ms / 1e3) if sieve_ms else None,
+            "prime_stream_GBps": primes * 8 / (sieve_ms / 1e3) / 1e9 if sieve_ms else None,
+            "sieve_ms": sieve_ms,
+            "tf_ms": tf_ms,
+        },
+    }
+
+    if not args.no_cpu:
+        threads = min(os.cpu_count() or 1, 16)
+        sample = 1 << args.cpu_sample_bits
+        f0s = F0 if F0 > 2 else 3
+        rate, cdt, cpr, ccounts, cfactors = cpu_baseline(seq, param, n0, n1, f0s, sample, threads)
+        # the same sample on the GPU: bit-exact check of the timed path
+        gc, gf, _ = eng.run_job(seq, param, n0, n1, f0s, f0s + sample, mx)
+        ok = bool(np.array_equal(gc, ccounts[: len(gc)]))
+        if ok:
+            for i in range(N):
+                k = min(int(gc[i]), mx)
+                if sorted(gf[i * mx:i * mx + k].tolist()) != sorted(cfactors[i * 1024:i * 1024 + k].tolist()):
+                    ok = False
+                    break
+        result["cpu_baseline"] = {
+            "value": rate,
+            "unit": "updates/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"oracle (C restatement of the reference CPU path) on primes in [{f0s}, {f0s + sample}) "
+                      f"({cpr} primes x {N} indices), {cdt:.1f} s with {threads} threads",
+            "gpu_matches_cpu_on_sample": ok,
+        }
+    print(json.dumps(result), flush=True)
+    eng.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -113,7 +113,25 @@ SIGNATURES = {
         ],
     ),
     "ctd_hip_last_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(CtdRunStats)]),
+    "ctd_hip_residues": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.POINTER(CtdJob), ctypes.POINTER(u64), u64, ctypes.POINTER(u64), u32],
+    ),
+    "ctd_hip_valu_probe": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, u32, ctypes.POINTER(ctypes.c_double)]),
 }
+
+PROBE_OPS = [
+    "v_add_u32",
+    "v_mul_lo_u32",
+    "v_mul_hi_u32",
+    "v_mad_u64_u32",
+    "v_mad_u32_u24",
+    "v_lshl_add_u64",
+    "v_cndmask_b32",
+    "v_alignbit_b32",
+    "v_min_u32",
+    "v_lshlrev_b64",
+]
 
 _lib = None
 

@@ -22,6 +22,7 @@
 #include "ctd_families.hpp"
 #include "ctd_sieve.hpp"
 #include "ctd_tf_fast.hpp"
+#include "ctd_probe.hpp"
 
 using namespace ctd;
 
@@ -49,6 +50,7 @@ __global__ void __launch_bounds__(256) tf_generic_kernel(JobArgs J, const uint64
             Gen<FAM>::step(s, M, n);
             record_hits(J, n + 1 - J.n0, p, Gen<FAM>::zero(s, M));
         }
+        if (J.final_res) J.final_res[i] = Gen<FAM>::plain(s, M);
     }
 }
 
@@ -61,13 +63,15 @@ __global__ void __launch_bounds__(256) tf_fast_kernel(JobArgs J, const uint64_t 
     const uint64_t work = tr * J.nchunks;
     for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < work;
          w += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t c = w / tr, i = w - c * tr;
+        // the chunk is wave-uniform (wave-aligned slots): keep it scalar
+        const uint64_t c = __builtin_amdgcn_readfirstlane((uint32_t)(w / tr));
+        const uint64_t i = w - c * tr;
         if (i >= nprimes) continue;
         const uint64_t p = primes[i];
         const uint64_t ns = J.n0 + c * J.chunk;
         uint64_t ne = ns + J.chunk;
         if (ne > J.n1 || ne < ns) ne = J.n1;
-        Fast<FAM>::run(J, p, ns, ne);
+        Fast<FAM>::run(J, p, ns, ne, i);
     }
 }
 
@@ -271,6 +275,21 @@ void build_base(ctd_hip_engine *E, uint64_t lim)
     E->base_lim = lim;
 }
 
+unsigned bitlen64(uint64_t x) { return x ? 64u - (unsigned)__builtin_clzll(x) : 0u; }
+
+// Conditions of the smarandache 32-bit-quotient step (ctd_tf_fast.hpp):
+// m = b^digits(n) < 2^21 for every appended n < n1, bitlen(p) <= 54,
+// bitlen(m) + bitlen(p) + 1 <= 64, and n1 <= 2^32 <= p (n+1 < p).
+bool sm_narrow_ok(uint64_t b, uint64_t n1, uint64_t pmax)
+{
+    if (n1 > (1ull << 32)) return false;
+    uint64_t m = b;
+    while (m <= n1 && m < (1ull << 22)) m *= b; // b^digits(n1)
+    if (m >= (1ull << 21)) return false;
+    unsigned k = bitlen64(pmax);
+    return k <= 54 && bitlen64(m) + k + 1 <= 64;
+}
+
 bool valid_job(ctd_hip_engine *E, const ctd_job *j, int *code)
 {
     *code = CTD_E_INVALID;
@@ -425,6 +444,7 @@ int run_job(ctd_hip_engine *E, const ctd_job *job, const ctd_run_opts *opts, con
                 }
                 J.chunk = chunk;
                 J.nchunks = (uint32_t)nch;
+                J.flags = (J.fam == FAM_SMARANDACHE && sm_narrow_ok(J.param, J.n1, bhi - 1)) ? JOB_SM_NARROW : 0u;
                 uint32_t block = (opts && opts->block_size > 0) ? (uint32_t)opts->block_size : 256u;
                 block = std::max<uint32_t>(64, std::min<uint32_t>(block, 256)) & ~63u;
                 uint64_t items = ((np + 63) & ~63ull) * nch;
@@ -617,6 +637,94 @@ int ctd_hip_trial_factor(ctd_hip_engine *E, const ctd_job *job, const ctd_run_op
         if (df) (void)hipFree(df);
         return fail_hip(E, h);
     }
+}
+
+int ctd_hip_residues(ctd_hip_engine *E, const ctd_job *job, const uint64_t *primes, uint64_t nprimes,
+                     uint64_t *out, uint32_t flags)
+{
+    if (!E) return CTD_E_INVALID;
+    int code;
+    if (!valid_job(E, job, &code)) return code;
+    if (!primes || !out || job->n1 <= job->n0) return fail(E, CTD_E_INVALID, "invalid residues request");
+    uint64_t pmax = 0;
+    for (uint64_t i = 0; i < nprimes; ++i) {
+        if (!(primes[i] & 1) || primes[i] < 3) return fail(E, CTD_E_INVALID, "moduli must be odd and >= 3");
+        pmax = std::max(pmax, primes[i]);
+    }
+    if (!nprimes) return CTD_OK;
+    uint32_t *dc = nullptr;
+    uint64_t *df = nullptr, *dr = nullptr;
+    try {
+        HIPCHK(hipSetDevice(E->device));
+        const uint64_t N = job->n1 - job->n0;
+        HIPCHK(hipMalloc((void **)&dc, N * sizeof(uint32_t)));
+        HIPCHK(hipMalloc((void **)&df, N * (uint64_t)job->max_factors * sizeof(uint64_t)));
+        HIPCHK(hipMalloc((void **)&dr, nprimes * sizeof(uint64_t)));
+        HIPCHK(hipMemsetAsync(dc, 0, N * sizeof(uint32_t), E->stream));
+        ensure_buf(E->d_primes, E->primes_cap, nprimes);
+        HIPCHK(hipMemcpyAsync(E->d_primes, primes, nprimes * sizeof(uint64_t), hipMemcpyHostToDevice, E->stream));
+        JobArgs J{};
+        J.fam = job->seq_id;
+        J.maxf = job->max_factors;
+        J.param = job->param;
+        J.n0 = job->n0;
+        J.n1 = job->n1;
+        J.chunk = N;
+        J.nchunks = 1;
+        J.counts = dc;
+        J.factors = df;
+        J.final_res = dr;
+        const bool fast = !(flags & CTD_RUN_GENERIC_ONLY) && Fast<0>::fits(pmax);
+        J.flags = (J.fam == FAM_SMARANDACHE && sm_narrow_ok(J.param, J.n1, pmax)) ? JOB_SM_NARROW : 0u;
+        const uint64_t items = (nprimes + 63) & ~63ull;
+        launch_tf_any(E, J, nprimes, fast, dim3((uint32_t)div_up(items, 256)), dim3(256), E->stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(out, dr, nprimes * sizeof(uint64_t), hipMemcpyDeviceToHost, E->stream));
+        HIPCHK(hipStreamSynchronize(E->stream));
+        HIPCHK(hipFree(dc));
+        HIPCHK(hipFree(df));
+        HIPCHK(hipFree(dr));
+    } catch (const HipError &h) {
+        if (dc) (void)hipFree(dc);
+        if (df) (void)hipFree(df);
+        if (dr) (void)hipFree(dr);
+        return fail_hip(E, h);
+    }
+    return CTD_OK;
+}
+
+int ctd_hip_valu_probe(ctd_hip_engine *E, int op, uint32_t iters, double *rate)
+{
+    if (!E || !rate || op < 0 || op >= PROBE_COUNT || iters == 0) return fail(E, CTD_E_INVALID, "invalid probe");
+    try {
+        HIPCHK(hipSetDevice(E->device));
+        uint32_t *d = nullptr;
+        HIPCHK(hipMalloc((void **)&d, sizeof(uint32_t)));
+        const uint32_t blocks = (uint32_t)E->cu * 16, threads = 256;
+        auto launch = [&](uint32_t it) {
+            switch (op) {
+#define CTD_PROBE_CASE(K)                                                                                     \
+    case K: hipLaunchKernelGGL(valu_probe_kernel<K>, dim3(blocks), dim3(threads), 0, E->stream, d, it); break;
+                CTD_PROBE_CASE(0) CTD_PROBE_CASE(1) CTD_PROBE_CASE(2) CTD_PROBE_CASE(3) CTD_PROBE_CASE(4)
+                CTD_PROBE_CASE(5) CTD_PROBE_CASE(6) CTD_PROBE_CASE(7) CTD_PROBE_CASE(8) CTD_PROBE_CASE(9)
+#undef CTD_PROBE_CASE
+            }
+        };
+        launch(1); // warm-up
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(E->ev0, E->stream));
+        launch(iters);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(E->ev1, E->stream));
+        HIPCHK(hipEventSynchronize(E->ev1));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, E->ev0, E->ev1));
+        HIPCHK(hipFree(d));
+        *rate = (double)blocks * threads * iters * PROBE_CHAINS * PROBE_DEPTH / (ms * 1e-3);
+    } catch (const HipError &h) {
+        return fail_hip(E, h);
+    }
+    return CTD_OK;
 }
 
 } // extern "C"

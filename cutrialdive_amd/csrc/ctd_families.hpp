@@ -38,6 +38,9 @@ enum Family : uint32_t {
     FAM_PEANO = 255
 };
 
+// JobArgs::flags
+constexpr uint32_t JOB_SM_NARROW = 1; // smarandache 32-bit-quotient step allowed (see ctd_tf_fast.hpp)
+
 // Job constants visible to every lane.
 struct JobArgs {
     uint32_t fam;
@@ -46,9 +49,10 @@ struct JobArgs {
     uint64_t n0, n1;    // index range
     uint64_t chunk;     // indices per work item
     uint32_t nchunks;
-    uint32_t pad;
+    uint32_t flags;     // JOB_* kernel selections made by the host
     uint32_t *counts;   // [n1-n0]
     uint64_t *factors;  // [(n1-n0) * maxf]
+    uint64_t *final_res; // test hook: S(ne-1) mod p per prime index, or null
 };
 
 // ---- S(n) mod 2 (prime 2 path) -------------------------------------------
@@ -248,6 +252,17 @@ struct Gen {
         } else { // test sequence S(n) = n
             s.x = n % p;
         }
+    }
+
+    // Plain residue S(n) mod p of the state at index n (test hook).
+    __device__ static uint64_t plain(const GenState &s, const Mont &M)
+    {
+        const uint64_t p = M.p;
+        if constexpr (FAM == FAM_MERSENNE || FAM == FAM_RIESEL) return s.x ? s.x - 1 : p - 1;
+        else if constexpr (FAM == FAM_PROTH) return s.x == p - 1 ? 0 : s.x + 1;
+        else if constexpr (FAM == FAM_FIBONACCI || FAM == FAM_LUCAS) return s.y;
+        else if constexpr (FAM == FAM_SMARANDACHE) return M.to_plain(s.x);
+        else return s.x;
     }
 
     __device__ static __forceinline__ bool zero(const GenState &s, const Mont &M)

@@ -244,6 +244,25 @@ class Engine:
         self._check(rc)
         return last.value
 
+    def residues(self, seq_id: int, param: int, n0: int, n1: int, moduli, flags: int = 0) -> np.ndarray:
+        """Test hook: S(n1-1) mod each modulus through the residue kernels."""
+        mods = np.ascontiguousarray(np.asarray(moduli, dtype=np.uint64))
+        out = np.zeros(len(mods), dtype=np.uint64)
+        job = nat.CtdJob(seq_id, 8, param, n0, n1, 3, 5)
+        P = ctypes.POINTER(nat.u64)
+        self._check(
+            self.lib.ctd_hip_residues(
+                self.h, ctypes.byref(job), mods.ctypes.data_as(P), len(mods), out.ctypes.data_as(P), flags
+            )
+        )
+        return out
+
+    def valu_probe(self, op: int, iters: int = 4096) -> float:
+        """Lane-instructions per second of probe op (see _native.PROBE_OPS)."""
+        rate = ctypes.c_double()
+        self._check(self.lib.ctd_hip_valu_probe(self.h, op, iters, ctypes.byref(rate)))
+        return rate.value
+
     def last_stats(self) -> nat.CtdRunStats:
         st = nat.CtdRunStats()
         self._check(self.lib.ctd_hip_last_stats(self.h, ctypes.byref(st)))

@@ -139,6 +139,25 @@ typedef struct ctd_run_stats {
 } ctd_run_stats;
 int ctd_hip_last_stats(const ctd_hip_engine *eng, ctd_run_stats *out);
 
+/* Test hook of the residue kernels: for each odd modulus primes[i] (host
+ * array), runs the same kernel path as ctd_hip_trial_factor over [n0, n1) and
+ * writes the plain residue S(n1-1) mod primes[i] to out[i] (host array).
+ * flags takes CTD_RUN_GENERIC_ONLY.  Hits are recorded into scratch buffers
+ * and discarded.  No reference counterpart: it exposes the per-prime value
+ * that trial_div_by_odd_primes (trial_factoring.cuh:156-228) keeps in
+ * registers, so every step of the propagation can be checked, not only the
+ * rare zero residues. */
+int ctd_hip_residues(ctd_hip_engine *eng, const ctd_job *job, const uint64_t *primes,
+                     uint64_t nprimes, uint64_t *out, uint32_t flags);
+
+/* VALU throughput probe: lane-instructions per second of one instruction
+ * (op: 0 v_add_u32, 1 v_mul_lo_u32, 2 v_mul_hi_u32, 3 v_mad_u64_u32,
+ * 4 v_mad_u32_u24, 5 v_lshl_add_u64, 6 v_cndmask_b32, 7 v_alignbit_b32,
+ * 8 v_min_u32, 9 v_lshlrev_b64) over independent chains on every CU.  The
+ * measured integer peak the roofline is quoted against.  No reference
+ * counterpart: the reference publishes no device peak. */
+int ctd_hip_valu_probe(ctd_hip_engine *eng, int op, uint32_t iters, double *lane_ops_per_s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -590,6 +590,75 @@ static uint64_t seq_next(const seqdef *s, uint64_t r, uint64_t n, const bctx *c,
     return 0;
 }
 
+static int seq_known(uint32_t seq);
+
+int ctdo_residues_last(uint32_t seq, uint64_t param, uint64_t n0, uint64_t count,
+                       const uint64_t *d, uint64_t nd, uint64_t *out)
+{
+    if (!seq_known(seq) || count == 0) return -1;
+    seqdef s = {seq, param, n0, NULL, 0};
+    uint64_t *limbs = NULL;
+    if (seq == CTDO_SMARANDACHE) {
+        uint64_t len = ctdo_smarandache_value(param, n0, NULL, 0);
+        limbs = malloc((len ? len : 1) * sizeof *limbs);
+        ctdo_smarandache_value(param, n0, limbs, len);
+        s.sm_limbs = limbs;
+        s.sm_len = len;
+    }
+    for (uint64_t k = 0; k < nd; ++k) {
+        if (!(d[k] & 1) || d[k] < 3) {
+            free(limbs);
+            return -1;
+        }
+        bctx c = bctx_make(d[k]);
+        fstate st;
+        uint64_t r = seq_init(&s, &c, &st);
+        for (uint64_t i = 1; i < count; ++i) r = seq_next(&s, r, n0 + i - 1, &c, &st);
+        out[k] = r;
+    }
+    free(limbs);
+    return 0;
+}
+
+uint64_t ctdo_next(uint32_t seq, uint64_t param, uint64_t r, uint64_t n, uint64_t d)
+{
+    seqdef s = {seq, param, 0, NULL, 0};
+    bctx c = bctx_make(d);
+    fstate st = {0, 0, 0};
+    return seq_next(&s, r, n, &c, &st);
+}
+
+int ctdo_state(uint32_t seq, uint64_t param, uint64_t n, uint64_t d, uint64_t out[3])
+{
+    bctx c = bctx_make(d);
+    fstate st;
+    switch (seq) {
+    case CTDO_FIBONACCI:
+        st = fib_state(n, &c);
+        out[0] = st.a;
+        out[1] = st.b;
+        return 2;
+    case CTDO_LUCAS: {
+        seqdef s = {seq, param, n, NULL, 0};
+        (void)seq_init(&s, &c, &st);
+        out[0] = st.a;
+        out[1] = st.b;
+        return 2;
+    }
+    case CTDO_TRIBONACCI:
+        st = trib_state(n, &c);
+        out[0] = st.a;
+        out[1] = st.b;
+        out[2] = st.c;
+        return 3;
+    case CTDO_PROTH_FIXED_N:
+    case CTDO_RIESEL_FIXED_N:
+        out[0] = modpow_b(2, param, &c);
+        return 1;
+    }
+    return 0;
+}
+
 static int seq_known(uint32_t seq)
 {
     return seq <= CTDO_TRIBONACCI || seq == CTDO_PEANO;

@@ -62,6 +62,18 @@ uint64_t ctdo_next_value_mod_2(uint32_t seq, uint64_t param, uint64_t r, uint64_
  * does (trial_factoring_impl.hpp:199-246): init once, then propagate.  d odd. */
 int ctdo_residues(uint32_t seq, uint64_t param, uint64_t n0, uint64_t count,
                   uint64_t d, uint64_t *out);
+/* For each odd modulus d[i]: out[i] = S(n0+count-1) mod d[i], propagated from
+ * n0 exactly as ctdo_residues does (the start value is built once). */
+int ctdo_residues_last(uint32_t seq, uint64_t param, uint64_t n0, uint64_t count,
+                       const uint64_t *d, uint64_t nd, uint64_t *out);
+/* One propagation step S(n+1) mod d from r = S(n) mod d for the stateless
+ * families (mersenne, proth, riesel, smarandache, test sequence): the
+ * next_value_mod[_mu] functions restated below. */
+uint64_t ctdo_next(uint32_t seq, uint64_t param, uint64_t r, uint64_t n, uint64_t d);
+/* Family state at index n modulo d (make_state): fibonacci/lucas write
+ * {previous, current}, tribonacci {T(n), T(n+1), T(n+2)}, fixed-n {2^n mod d}.
+ * Returns the number of words written (0 for stateless families). */
+int ctdo_state(uint32_t seq, uint64_t param, uint64_t n, uint64_t d, uint64_t out[3]);
 /* Smarandache base b: S(n) as little-endian 64-bit limbs (the HgInt value of
  * smarandache.cpp:129-158).  Returns the limb count; writes at most cap. */
 uint64_t ctdo_smarandache_value(uint64_t base, uint64_t n, uint64_t *limbs, uint64_t cap);

@@ -65,6 +65,9 @@ def lib() -> ctypes.CDLL:
             "ctdo_next_value_mod_2": (u64, [u32, u64, u64, u64]),
             "ctdo_residues": (ctypes.c_int, [u32, u64, u64, u64, u64, P64]),
             "ctdo_smarandache_value": (u64, [u64, u64, P64, u64]),
+            "ctdo_next": (u64, [u32, u64, u64, u64, u64]),
+            "ctdo_residues_last": (ctypes.c_int, [u32, u64, u64, u64, P64, u64, P64]),
+            "ctdo_state": (ctypes.c_int, [u32, u64, u64, u64, P64]),
             "ctdo_trial_factor": (ctypes.c_int, [ctypes.POINTER(Job), u32, ctypes.c_void_p, ctypes.c_void_p]),
             "ctdo_job_primes": (u64, [ctypes.POINTER(Job)]),
         }
@@ -99,6 +102,25 @@ def residues(seq: int, param: int, n0: int, count: int, d: int) -> List[int]:
     if rc != 0:
         raise ValueError("invalid residues request")
     return [int(x) for x in out]
+
+
+def residues_last(seq: int, param: int, n0: int, count: int, moduli) -> np.ndarray:
+    """S(n0+count-1) mod each modulus, by propagation from n0."""
+    d = np.ascontiguousarray(np.asarray(moduli, dtype=np.uint64))
+    out = np.zeros(len(d), dtype=np.uint64)
+    if lib().ctdo_residues_last(seq, param, n0, count, _p64(d), len(d), _p64(out)) != 0:
+        raise ValueError("invalid residues request")
+    return out
+
+
+def next_residue(seq: int, param: int, r: int, n: int, d: int) -> int:
+    return int(lib().ctdo_next(seq, param, r, n, d))
+
+
+def state(seq: int, param: int, n: int, d: int) -> List[int]:
+    out = np.zeros(3, dtype=np.uint64)
+    k = lib().ctdo_state(seq, param, n, d, _p64(out))
+    return [int(x) for x in out[:k]]
 
 
 def smarandache_value(base: int, n: int) -> int:

@@ -153,11 +153,13 @@ def test_cfg1_reduced(engine):
 
 
 def test_cfg2_subwindow(engine):
-    # Sm10 n in [10, 1000) over [2^32, 2^32 + 2^22) (config 2 sub-window)
-    f0, f1 = 1 << 32, (1 << 32) + (1 << 22)
-    g, _, _ = gpu_lists(engine, o.SMARANDACHE, 10, 10, 1000, f0, f1)
-    r = o.trial_factor(o.SMARANDACHE, 10, 10, 1000, f0, f1)
+    # Sm10 n in [10, 1000) over [2^32, 2^32 + 2^28) (config 2 sub-window,
+    # 12.4 M primes); it holds zero residues, so the lists are not trivially empty
+    f0, f1 = 1 << 32, (1 << 32) + (1 << 28)
+    g, gc, _ = gpu_lists(engine, o.SMARANDACHE, 10, 10, 1000, f0, f1)
+    r = o.trial_factor(o.SMARANDACHE, 10, 10, 1000, f0, f1, threads=16)
     assert g == r
+    assert int(gc.sum()) >= 3
 
 
 def test_cfg3_subwindow(engine):

@@ -1,0 +1,88 @@
+"""GPU residue parity: the value each lane carries after propagating over the
+index range, S(n1-1) mod p, must equal the oracle's for every prime.
+
+Zero residues (the factors) are rare for large primes, so factor-list
+comparisons alone leave most of the arithmetic unobserved; the final residue
+exposes every step.  Each case runs the fast kernel and the generic
+(64-bit Montgomery) kernel.
+"""
+import random
+
+import numpy as np
+import pytest
+import sympy
+
+from cutrialdive_amd import _native as nat
+from oracle import oracle as o
+
+pytestmark = pytest.mark.gpu
+
+FAMILIES = [
+    (o.MERSENNE, 0),
+    (o.PROTH, 3),
+    (o.PROTH, 2718281828459),
+    (o.RIESEL, 37),
+    (o.RIESEL, (1 << 64) - 59),
+    (o.PROTH_FIXED_N, 100000),
+    (o.RIESEL_FIXED_N, 777),
+    (o.FIBONACCI, 0),
+    (o.LUCAS, 0),
+    (o.TRIBONACCI, 0),
+    (o.SMARANDACHE, 10),
+    (o.SMARANDACHE, 16),
+    (o.SMARANDACHE, 2),
+    (o.SMARANDACHE, 3),
+    (o.SMARANDACHE, 7),
+    (o.SMARANDACHE, 64),
+    (o.PEANO, 0),
+]
+
+# prime-size classes: 32-bit residues, 2^31..2^32, the narrow Smarandache
+# range, the config-3/4 range and near 2^63 / 2^64 (generic only above 2^63)
+WINDOWS = [(3, 1 << 20), (1 << 31, 1 << 32), (1 << 32, 1 << 36), (1 << 38, 1 << 43), (1 << 62, 1 << 63),
+           ((1 << 63) + 1, (1 << 64) - 1)]
+
+
+def random_primes(lo, hi, k, seed):
+    rng = random.Random(seed)
+    ps = set()
+    while len(ps) < k:
+        ps.add(int(sympy.randprime(lo, hi)) if hi - lo > 1000 else int(sympy.nextprime(rng.randrange(lo, hi))))
+    return sorted(p for p in ps if p > 2)
+
+
+@pytest.mark.parametrize("seq,param", FAMILIES)
+@pytest.mark.parametrize("win", range(len(WINDOWS)))
+def test_final_residues(engine, seq, param, win):
+    lo, hi = WINDOWS[win]
+    primes = random_primes(lo, hi, 256, seed=seq * 97 + win)
+    n0, n1 = (5, 2100) if seq != o.PEANO else (10 ** 12, 10 ** 12 + 500)
+    want = o.residues_last(seq, param, n0, n1 - n0, primes)
+    for flags in (0, nat.CTD_RUN_GENERIC_ONLY):
+        got = engine.residues(seq, param, n0, n1, primes, flags)
+        bad = np.nonzero(got != want)[0]
+        assert len(bad) == 0, (flags, [(primes[i], int(got[i]), int(want[i])) for i in bad[:5]])
+
+
+@pytest.mark.parametrize("seq,param", FAMILIES)
+def test_final_residues_jump_ahead(engine, seq, param):
+    # start far from 1: the jump-ahead initial state must equal propagation
+    primes = random_primes(1 << 40, 1 << 41, 128, seed=seq + 5)
+    n0, n1 = 98765, 100321
+    want = o.residues_last(seq, param, n0, n1 - n0, primes)
+    got = engine.residues(seq, param, n0, n1, primes, 0)
+    assert np.array_equal(got, want)
+
+
+def test_smarandache_base10_digit_boundaries(engine):
+    # every digit-length change from 1 to 6 digits inside one run
+    primes = random_primes(1 << 32, 1 << 34, 256, seed=1)
+    for n0, n1 in ((1, 120), (95, 1005), (9990, 10011), (99990, 100012), (1, 20000)):
+        want = o.residues_last(o.SMARANDACHE, 10, n0, n1 - n0, primes)
+        got = engine.residues(o.SMARANDACHE, 10, n0, n1, primes, 0)
+        assert np.array_equal(got, want), (n0, n1)
+
+
+def test_residues_rejects_even_moduli(engine):
+    with pytest.raises(RuntimeError):
+        engine.residues(o.MERSENNE, 0, 1, 10, [4], 0)
