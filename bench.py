@@ -38,13 +38,15 @@ FAMILY_NAMES = {0: "mersenne", 1: "proth", 2: "riesel", 3: "smarandache", 4: "fi
 METRIC = "prime×index residue updates/s, batched vs plain on same GPU, bit-exact factors"
 
 
-def ops_per_update(seq: int, param: int, pmax: int) -> float:
+def ops_per_update(workload: str, seq: int, pmax: int):
+    """(VALU lane-ops per update, HBM read bytes per prime, source) for the workload."""
     with open(os.path.join(ROOT, "cutrialdive_amd", "ops_per_update.json")) as f:
         t = json.load(f)
-    fam = t[FAMILY_NAMES[seq]]
-    if seq == 3 and (param & (param - 1)) == 0:
-        return float(fam.get("w64_pow2_base", fam["w64"]))
-    return float(fam["w32"] if pmax < (1 << 31) else fam["w64"])
+    m = t["measured"].get(workload)
+    if m:
+        return float(m["valu_lane_ops_per_update"]), float(m["hbm_read_bytes_per_prime"]), "rocprofv3 " + m["profile"]
+    fam = t["loop"][FAMILY_NAMES[seq]]
+    return float(fam["w32"] if pmax < (1 << 31) else fam["w64"]), 8.0, "loop ISA count"
 
 
 def cpu_baseline(seq, param, n0, n1, f0, sample_len, threads):
@@ -123,8 +125,9 @@ def main() -> int:
     dt = time.perf_counter() - t0
 
     value = updates / dt
-    c_upd = ops_per_update(seq, param, F1 - 1)
-    peak = 39.3  # T int32 lane-ops/s
+    c_upd, read_bpp, c_src = ops_per_update(args.workload, seq, F1 - 1)
+    # measured VALU peak: independent v_add_u32 chains on every CU (same run)
+    peak = eng.valu_probe(0, 4096) / 1e12
     avg_launch_s = tf_ms / 1e3 / max(launches, 1)
     per_launch_ops = updates / max(launches, 1) * c_upd
     achieved = per_launch_ops / avg_launch_s / 1e12
@@ -157,11 +160,15 @@ def main() -> int:
             "bound": "valu",
             "achieved": achieved,
             "peak": peak,
-            "unit": "T int32 lane-ops/s",
+            "unit": "T int32 VALU lane-ops/s",
             "frac": achieved / peak,
-            "traffic": None,
+            "traffic": read_bpp * primes / max(launches, 1),
+            "traffic_unit": "HBM bytes per launch (prime stream read; FETCH_SIZE x2 calibration)",
             "kernel": f"tf_fast_kernel<{FAMILY_NAMES[seq]}>",
             "ops_per_update": c_upd,
+            "ops_per_update_source": c_src,
+            "peak_source": "v_add_u32 independent-chain probe, measured in this run",
+            "nominal_peak": 39.3,
             "avg_launch_ms": avg_launch_s * 1e3,
         },
         "sieve": {

@@ -511,7 +511,7 @@ int ctd_hip_open(int device, ctd_hip_engine **out)
         HIPCHK(hipSetDevice(device));
         hipDeviceProp_t prop;
         HIPCHK(hipGetDeviceProperties(&prop, device));
-        E->name = prop.name;
+        E->name = prop.name[0] ? prop.name : prop.gcnArchName;
         E->total_mem = prop.totalGlobalMem;
         E->cu = prop.multiProcessorCount;
         HIPCHK(hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking));
