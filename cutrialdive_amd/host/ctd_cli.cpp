@@ -315,8 +315,8 @@ int autotest()
         {256, 307, 8, 257, 293},
         {65536, 65537, 0, 0, 0},
         {65536, 65538, 1, 65537, 65537},
-        {1ull << 32, (1ull << 32) + 1000, 56, 4294967311ull, 4294968241ull},
-        {(1ull << 61) - 100, 1ull << 61, 3, 2305843009213693921ull, 2305843009213693951ull},
+        {1ull << 32, (1ull << 32) + 1000, 56, 4294967311ull, 4294968289ull},
+        {(1ull << 61) - 100, 1ull << 61, 3, 2305843009213693907ull, 2305843009213693951ull},
         {0, 1ull << 17, 12251, 2, 131071},
     }};
     int failures = 0;

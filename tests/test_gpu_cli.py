@@ -23,6 +23,13 @@ def text_of(lists, n0):
     return "".join(f"{n0 + i} - {', '.join(str(p) for p in fs)}\n" for i, fs in enumerate(lists))
 
 
+def same_text(got: str, expect: str):
+    g, e = got.splitlines(), expect.splitlines()
+    bad = [(a, b) for a, b in zip(g, e) if a != b]
+    assert len(g) == len(e) and not bad, (len(g), len(e), bad[:3])
+    assert got == expect
+
+
 def cli(*args, timeout=120):
     print("cli:", *args, flush=True)
     r = subprocess.run([CLI, *map(str, args)], capture_output=True, text=True, timeout=timeout)
@@ -37,7 +44,7 @@ def cli(*args, timeout=120):
     ("smarandache", 3, 10, 10, 40, 2, 1 << 28),
     ("riesel,5", 2, 5, 1, 300, 1 << 20, (1 << 20) + (1 << 22)),
     ("fibonacci", 4, 0, 1, 500, 3, 1 << 22),
-    ("lucas", 5, 0, 0, 500, 2, 1 << 22),
+    ("lucas", 5, 0, 1, 500, 2, 1 << 22),
     ("tribonacci", 8, 0, 2, 400, 2, 1 << 22),
     ("proth_fixed_n,100", 6, 100, 1, 300, 2, 1 << 22),
     ("riesel_fixed_n,33", 7, 33, 2, 300, 2, 1 << 22),
@@ -48,7 +55,7 @@ def test_output_file_matches_oracle(tmp_path, spec, seq, param, n0, n1, f0, f1):
     cli("--num-seq", spec, "-s", n0, "-e", n1, "--tf-start", f0, "--tf-end", f1, "--no-progress",
         "--max-factors-per-number", 1024, "--output", out)
     expect = text_of(oracle.trial_factor(seq, param, n0, n1, f0, f1, 1024), n0)
-    assert out.read_text() == expect
+    same_text(out.read_text(), expect)
     assert not os.path.exists(str(out) + ".chkpnt")
 
 
@@ -80,7 +87,7 @@ def test_resume_equals_uninterrupted_run(tmp_path):
     r = cli("--resume", ck, "--no-progress", "--checkpoint-period", "1")
     assert f"Resuming at the smallest prime > {last}." in r.stdout
     expect = text_of(oracle.trial_factor(seq, param, n0, n1, f0, f1, mx), n0)
-    assert out.read_text() == expect
+    same_text(out.read_text(), expect)
     assert not ck.exists()
 
 
@@ -104,3 +111,8 @@ def test_resume_keeps_counts_beyond_max(tmp_path):
         assert set(got) <= set(full[i])
         # the factors stored in the checkpoint are kept
         assert set(rows[i][1]) <= set(got)
+
+
+def test_autotest_passes():
+    r = cli("--autotest")
+    assert "All self tests passed." in r.stdout
