@@ -1,18 +1,29 @@
 #!/usr/bin/env python3
 """Prime x index residue updates/s of the MI355X trial-factoring engine.
 
-Default workload (config 2 of the baseline): Smarandache base 10,
-n in [10, 1000), every prime in [2^32, 2^36), plain single-job form.
-One step is one device batch: 2^31 consecutive integers of the prime window
-sieved on the device and every prime of it propagated over all 990 indices.
-The default 30 timed steps cover the whole window [2^32, 2^36) once.
+Default workload ("mixed"): the seeded batch of 256 uneven jobs
+(cutrialdive_amd/workloads.py: proth/riesel k, proth_fixed_n, fibonacci,
+lucas, tribonacci, smarandache b in 2..16; 10^2..10^5 indices each; a 2^30-wide
+prime window in [2^38, 2^43) each), about 1.3e14 updates.  One step runs the
+whole batch in the batched form (one hipGraph launch, autotuned launch shapes,
+factors buffers resident in HBM).  The same jobs then run in the plain form,
+one ctd_hip_trial_factor_device call per job, for the batched-vs-plain ratio,
+and the two sets of factors buffers are compared.
 
-Prints one JSON line (see DESIGN.md for the field definitions).
+Other workloads (--workload): single jobs of the plain form, one step per
+device batch of 2^31 integers of the prime window:
+  cfg2   Sm10, n in [10, 1000), primes in [2^32, 2^36) (30 steps = the window)
+  cfg1   Mersenne, n in [2, 20000), primes < 2^30
+  cfg3a  Sm16, n in [1, 2000), primes in [2^40, 2^40 + 2^36)
+  cfg3b  Proth k=3, n in [1, 10^5), primes in [2^40, 2^40 + 2^32)
+
+Prints one JSON line.
 """
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -22,7 +33,7 @@ sys.path.insert(0, ROOT)
 
 import numpy as np  # noqa: E402
 
-WORKLOADS = {
+SINGLE = {
     # name: (seq_id, param, n0, n1, F0, F1, batch, description)
     "cfg2": (3, 10, 10, 1000, 1 << 32, 1 << 36, 1 << 31,
              "smarandache,10 n in [10,1000), primes in [2^32,2^36), plain single-job form"),
@@ -36,61 +47,228 @@ WORKLOADS = {
 FAMILY_NAMES = {0: "mersenne", 1: "proth", 2: "riesel", 3: "smarandache", 4: "fibonacci", 5: "lucas",
                 6: "proth_fixed_n", 7: "riesel_fixed_n", 8: "tribonacci"}
 METRIC = "prime×index residue updates/s, batched vs plain on same GPU, bit-exact factors"
+HBM_PEAK_GBPS = 8000.0
+
+
+def ops_table():
+    with open(os.path.join(ROOT, "cutrialdive_amd", "ops_per_update.json")) as f:
+        return json.load(f)
 
 
 def ops_per_update(workload: str, seq: int, pmax: int):
-    """(VALU lane-ops per update, HBM read bytes per prime, source) for the workload."""
-    with open(os.path.join(ROOT, "cutrialdive_amd", "ops_per_update.json")) as f:
-        t = json.load(f)
+    """(VALU lane-ops per update, source) for a single-job workload."""
+    t = ops_table()
     m = t["measured"].get(workload)
     if m:
-        return float(m["valu_lane_ops_per_update"]), float(m["hbm_read_bytes_per_prime"]), "rocprofv3 " + m["profile"]
+        return float(m["valu_lane_ops_per_update"]), "rocprofv3 " + m["profile"]
     fam = t["loop"][FAMILY_NAMES[seq]]
-    return float(fam["w32"] if pmax < (1 << 31) else fam["w64"]), 8.0, "loop ISA count"
+    return float(fam["w32"] if pmax < (1 << 31) else fam["w64"]), "loop ISA count"
 
 
-def cpu_baseline(seq, param, n0, n1, f0, sample_len, threads):
+def mixed_ops(jobs):
+    """Update-weighted VALU lane-ops per update of the mixed batch."""
+    t = ops_table()
+    m = t["measured"].get("mixed")
+    if m:
+        return float(m["valu_lane_ops_per_update"]), "rocprofv3 " + m["profile"]
+    num = den = 0.0
+    for j in jobs:
+        u = (j.f1 - j.f0) / math.log(j.f1) * (j.n1 - j.n0)
+        num += u * float(t["loop"][FAMILY_NAMES[j.seq_id]]["w64"])
+        den += u
+    return num / den, "loop ISA count, update-weighted"
+
+
+def cpu_sample(jobs, sample_len, threads):
+    """Oracle (the C restatement of the reference CPU path) on the first
+    sample_len integers of every job's window."""
     from oracle import oracle as o
 
-    f1 = f0 + sample_len
     t0 = time.perf_counter()
-    counts, factors = o.trial_factor_raw(seq, param, n0, n1, f0, f1, 1024, threads)
+    out = []
+    upd = 0
+    for j in jobs:
+        c, f = o.trial_factor_raw(j.seq_id, j.param, j.n0, j.n1, j.f0, j.f0 + sample_len, j.max_factors, threads)
+        out.append((c, f))
+        upd += o.job_primes(j.seq_id, j.param, j.n0, j.n1, j.f0, j.f0 + sample_len) * (j.n1 - j.n0)
     dt = time.perf_counter() - t0
-    primes = o.job_primes(seq, param, n0, n1, f0, f1)
-    return primes * (n1 - n0) / dt, dt, primes, counts, factors
+    return out, upd, dt
 
 
-def main() -> int:
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="cfg2", choices=sorted(WORKLOADS))
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-sample-bits", type=int, default=29)
-    ap.add_argument("--max-factors", type=int, default=1024)
-    args = ap.parse_args()
+def same_buffers(c, f, rc, rf, mx):
+    if not np.array_equal(np.asarray(c), np.asarray(rc)):
+        return False
+    for i, k in enumerate(np.asarray(c).tolist()):
+        if 0 < k <= mx and sorted(f[i * mx:i * mx + k].tolist()) != sorted(rf[i * mx:i * mx + k].tolist()):
+            return False
+    return True
 
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (--gpus {args.gpus} requested); the engine shards by prime "
-              "window across GPUs with no exchange in the hot loop, and multi-GPU runs are not part of this build.",
-              file=sys.stderr)
-        return 2
 
-    import torch
+def run_mixed(args, torch, ctd, nat):
+    from cutrialdive_amd import workloads
 
-    import cutrialdive_amd as ctd
-    from cutrialdive_amd import _native as nat
+    jobs = workloads.mixed_jobs(args.jobs)
+    eng = ctd.Engine(0)
+    name, mem, cu = eng.device_info()
+    Ns = [j.n1 - j.n0 for j in jobs]
+    mx = jobs[0].max_factors
+    print(f"# device {name}, {cu} CUs, {mem / 2**30:.0f} GiB; {len(jobs)} mixed jobs, "
+          f"{sum(Ns)} indices, ~{workloads.updates_of(jobs):.3e} updates", file=sys.stderr, flush=True)
+    offs = np.concatenate([[0], np.cumsum(Ns)])
+    d_counts = torch.zeros(int(offs[-1]), dtype=torch.int32, device="cuda")
+    d_factors = torch.zeros(int(offs[-1]) * mx, dtype=torch.int64, device="cuda")
+    cptr = [d_counts.data_ptr() + 4 * int(offs[k]) for k in range(len(jobs))]
+    fptr = [d_factors.data_ptr() + 8 * mx * int(offs[k]) for k in range(len(jobs))]
+    cjobs = [nat.CtdJob(j.seq_id, j.max_factors, j.param, j.n0, j.n1, j.f0, j.f1) for j in jobs]
+    flags = nat.CTD_BATCH_NO_AUTOTUNE if args.no_autotune else 0
 
-    seq, param, n0, n1, F0, F1, batch, desc = WORKLOADS[args.workload]
+    # warm-up: launch-shape probes for every (family, kind, index class), then
+    # a batch on the first 2^24 integers of every window
+    t0 = time.perf_counter()
+    warm = [nat.CtdJob(j.seq_id, j.max_factors, j.param, j.n0, j.n1, j.f0, j.f0 + (1 << 24)) for j in jobs]
+    for _ in range(args.warmup):
+        wst = eng.run_batch_raw(warm, cptr, fptr, flags, args.tile_bits and 1 << args.tile_bits)
+    tune_s = time.perf_counter() - t0
+    shapes = {}
+    if not args.no_autotune:
+        for j, c in zip(jobs, cjobs):
+            shapes.setdefault(f"{j.kind}/N{'<1k' if j.n1 - j.n0 < 1024 else '<16k' if j.n1 - j.n0 < 16384 else '>=16k'}",
+                              eng.batch_shape(c))
+    d_counts.zero_()
+    d_factors.zero_()
+    torch.cuda.synchronize()
+    print(f"# warm-up + autotune {tune_s:.1f} s", file=sys.stderr, flush=True)
+
+    # batched form (timed)
+    upd = primes = 0
+    bms = build = 0.0
+    graphs = tiles = 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        if k:
+            d_counts.zero_()
+            d_factors.zero_()
+        st = eng.run_batch_raw(cjobs, cptr, fptr, flags, args.tile_bits and 1 << args.tile_bits)
+        upd += st.updates
+        primes += st.primes
+        bms += st.total_ms
+        build += st.build_ms
+        graphs += st.graphs
+        tiles += st.tiles
+        print(f"# batched step {k}: {st.tiles} tiles, {st.graphs} graph(s), graph {st.total_ms:.0f} ms, "
+              f"build {st.build_ms:.0f} ms", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    value = upd / dt
+    bat_counts = d_counts.cpu().numpy().view(np.uint32)
+    bat_factors = d_factors.cpu().numpy().view(np.uint64)
+
+    # plain form: one job per call, same buffers
+    result_plain = None
+    if not args.no_plain:
+        d_counts.zero_()
+        d_factors.zero_()
+        torch.cuda.synchronize()
+        p_upd = 0
+        t0 = time.perf_counter()
+        for k, c in enumerate(cjobs):
+            eng.run_job_device(c, cptr[k], fptr[k])
+            p_upd += eng.last_stats().updates
+        torch.cuda.synchronize()
+        pdt = time.perf_counter() - t0
+        pc = d_counts.cpu().numpy().view(np.uint32)
+        pf = d_factors.cpu().numpy().view(np.uint64)
+        match = all(same_buffers(bat_counts[offs[k]:offs[k + 1]], bat_factors[offs[k] * mx:offs[k + 1] * mx],
+                                 pc[offs[k]:offs[k + 1]], pf[offs[k] * mx:offs[k + 1] * mx], mx)
+                    for k in range(len(jobs)))
+        result_plain = {"value": p_upd / pdt, "seconds": pdt, "updates": p_upd, "launch_form": "one job per call",
+                        "factors_identical_to_batched": bool(match)}
+        print(f"# plain form {pdt:.1f} s, identical factors: {match}", file=sys.stderr, flush=True)
+
+    c_upd, c_src = mixed_ops(jobs)
+    peak = eng.valu_probe(0, 4096) / 1e12
+    per_launch_s = bms / 1e3 / max(graphs, 1)
+    achieved = upd / max(graphs, 1) * c_upd / per_launch_s / 1e12
+    traffic = 16.0 * primes / max(graphs, 1)  # sieve writes 8 B per prime, the residue kernel reads 8 B
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3 / max(args.steps, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic (seeded job generator, seed %d)" % workloads.SEED,
+        "config": {
+            "workload": f"mixed batch of {len(jobs)} seeded jobs (7 families, 10^2..10^5 indices, "
+                        "2^30-wide prime windows in [2^38, 2^43)), batched form, one hipGraph launch",
+            "jobs": len(jobs),
+            "indices": int(sum(Ns)),
+            "primes_timed": primes,
+            "updates_timed": upd,
+            "tiles": tiles // max(args.steps, 1),
+            "graph_launches_per_step": graphs // max(args.steps, 1),
+            "graph_build_ms_per_step": build / max(args.steps, 1),
+            "autotune": not args.no_autotune,
+            "shapes": {k: {"block": b, "waves_per_cu": w, "chunk_split": s} for k, (b, w, s) in sorted(shapes.items())},
+        },
+        "plain": result_plain,
+        "batched_vs_plain": (value / result_plain["value"]) if result_plain else None,
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved,
+            "peak": peak,
+            "unit": "T int32 VALU lane-ops/s",
+            "frac": achieved / peak,
+            "traffic": traffic,
+            "traffic_unit": "HBM bytes per graph launch (prime stream: sieve write + residue read, 8 B each)",
+            "kernel": "batch graph (sieve_kernel + tf_fast_kernel<family> per tile)",
+            "ops_per_update": c_upd,
+            "ops_per_update_source": c_src,
+            "peak_source": "v_add_u32 independent-chain probe, measured in this run",
+            "nominal_peak": 39.3,
+            "avg_launch_ms": per_launch_s * 1e3,
+            "hbm": {"achieved_GBps": traffic / per_launch_s / 1e9, "peak_GBps": HBM_PEAK_GBPS,
+                    "frac": traffic / per_launch_s / 1e9 / HBM_PEAK_GBPS},
+        },
+    }
+    if not args.no_cpu:
+        threads = min(os.cpu_count() or 1, 16)
+        sample = 1 << args.cpu_sample_bits
+        ref, cupd, cdt = cpu_sample(jobs, sample, threads)
+        sjobs = [nat.CtdJob(j.seq_id, j.max_factors, j.param, j.n0, j.n1, j.f0, j.f0 + sample) for j in jobs]
+        res, _ = eng.run_batch(sjobs, flags=flags)
+        ok = all(same_buffers(c, f, rc, rf, mx) for (c, f), (rc, rf) in zip(res, ref))
+        result["cpu_baseline"] = {
+            "value": cupd / cdt,
+            "unit": "updates/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"oracle (C restatement of the reference CPU path) on the first 2^{args.cpu_sample_bits} "
+                      f"integers of each of the {len(jobs)} job windows ({cupd:.3e} updates), {cdt:.1f} s "
+                      f"with {threads} threads",
+            "gpu_matches_cpu_on_sample": bool(ok),
+        }
+    print(json.dumps(result), flush=True)
+    eng.close()
+    return 0
+
+
+def run_single(args, torch, ctd, nat):
+    seq, param, n0, n1, F0, F1, batch, desc = SINGLE[args.workload]
     N = n1 - n0
     mx = args.max_factors
     nbatches = (F1 - F0 + batch - 1) // batch
+    steps = args.steps if args.steps is not None else 30
     eng = ctd.Engine(0)
     name, mem, cu = eng.device_info()
     print(f"# device {name}, {cu} CUs, {mem / 2**30:.0f} GiB; workload {args.workload}: {desc}", file=sys.stderr,
           flush=True)
-
     d_counts = torch.zeros(N, dtype=torch.int32, device="cuda")
     d_factors = torch.zeros(N * mx, dtype=torch.int64, device="cuda")
     torch.cuda.synchronize()
@@ -107,12 +285,11 @@ def main() -> int:
     d_counts.zero_()
     d_factors.zero_()
     torch.cuda.synchronize()
-
     updates = primes = 0
     tf_ms = sieve_ms = 0.0
     launches = 0
     t0 = time.perf_counter()
-    for k in range(args.steps):
+    for k in range(steps):
         st = run_batch(k)
         updates += st.updates
         primes += st.primes
@@ -123,46 +300,35 @@ def main() -> int:
               flush=True)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-
     value = updates / dt
-    c_upd, read_bpp, c_src = ops_per_update(args.workload, seq, F1 - 1)
-    # measured VALU peak: independent v_add_u32 chains on every CU (same run)
+    c_upd, c_src = ops_per_update(args.workload, seq, F1 - 1)
     peak = eng.valu_probe(0, 4096) / 1e12
     avg_launch_s = tf_ms / 1e3 / max(launches, 1)
-    per_launch_ops = updates / max(launches, 1) * c_upd
-    achieved = per_launch_ops / avg_launch_s / 1e12
+    achieved = updates / max(launches, 1) * c_upd / avg_launch_s / 1e12
+    traffic = 8.0 * primes / max(launches, 1)
     result = {
         "metric": METRIC,
         "value": value,
         "unit": "updates/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": dt * 1e3 / args.steps,
+        "ms_per_step": dt * 1e3 / steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u64",
         "data": "synthetic",
-        "config": {
-            "workload": desc,
-            "sequence": FAMILY_NAMES[seq],
-            "param": param,
-            "n0": n0,
-            "n1": n1,
-            "prime_window": [F0, F1],
-            "batch_integers": batch,
-            "primes_timed": primes,
-            "updates_timed": updates,
-            "form": "plain",
-        },
+        "config": {"workload": desc, "sequence": FAMILY_NAMES[seq], "param": param, "n0": n0, "n1": n1,
+                   "prime_window": [F0, F1], "batch_integers": batch, "primes_timed": primes,
+                   "updates_timed": updates, "form": "plain"},
         "roofline": {
             "bound": "valu",
             "achieved": achieved,
             "peak": peak,
             "unit": "T int32 VALU lane-ops/s",
             "frac": achieved / peak,
-            "traffic": read_bpp * primes / max(launches, 1),
+            "traffic": traffic,
             "traffic_unit": "HBM bytes per launch (prime stream read; FETCH_SIZE x2 calibration)",
             "kernel": f"tf_fast_kernel<{FAMILY_NAMES[seq]}>",
             "ops_per_update": c_upd,
@@ -170,6 +336,8 @@ def main() -> int:
             "peak_source": "v_add_u32 independent-chain probe, measured in this run",
             "nominal_peak": 39.3,
             "avg_launch_ms": avg_launch_s * 1e3,
+            "hbm": {"achieved_GBps": traffic / avg_launch_s / 1e9, "peak_GBps": HBM_PEAK_GBPS,
+                    "frac": traffic / avg_launch_s / 1e9 / HBM_PEAK_GBPS},
         },
         "sieve": {
             "primes_per_s": primes / (sieve_ms / 1e3) if sieve_ms else None,
@@ -178,33 +346,67 @@ def main() -> int:
             "tf_ms": tf_ms,
         },
     }
-
     if not args.no_cpu:
+        from oracle import oracle as o
+
         threads = min(os.cpu_count() or 1, 16)
         sample = 1 << args.cpu_sample_bits
         f0s = F0 if F0 > 2 else 3
-        rate, cdt, cpr, ccounts, cfactors = cpu_baseline(seq, param, n0, n1, f0s, sample, threads)
-        # the same sample on the GPU: bit-exact check of the timed path
-        gc, gf, _ = eng.run_job(seq, param, n0, n1, f0s, f0s + sample, mx)
-        ok = bool(np.array_equal(gc, ccounts[: len(gc)]))
-        if ok:
-            for i in range(N):
-                k = min(int(gc[i]), mx)
-                if sorted(gf[i * mx:i * mx + k].tolist()) != sorted(cfactors[i * 1024:i * 1024 + k].tolist()):
-                    ok = False
-                    break
+        t0 = time.perf_counter()
+        ccounts, cfactors = o.trial_factor_raw(seq, param, n0, n1, f0s, f0s + sample, 1024, threads)
+        cdt = time.perf_counter() - t0
+        cpr = o.job_primes(seq, param, n0, n1, f0s, f0s + sample)
+        gc, gf, _ = eng.run_job(seq, param, n0, n1, f0s, f0s + sample, 1024)
+        ok = same_buffers(gc, gf, ccounts, cfactors, 1024)
         result["cpu_baseline"] = {
-            "value": rate,
+            "value": cpr * N / cdt,
             "unit": "updates/s",
             "cores": threads,
             "kind": "port",
             "sample": f"oracle (C restatement of the reference CPU path) on primes in [{f0s}, {f0s + sample}) "
                       f"({cpr} primes x {N} indices), {cdt:.1f} s with {threads} threads",
-            "gpu_matches_cpu_on_sample": ok,
+            "gpu_matches_cpu_on_sample": bool(ok),
         }
     print(json.dumps(result), flush=True)
     eng.close()
     return 0
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (mixed: 1, single-job workloads: 30)")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="mixed", choices=["mixed"] + sorted(SINGLE))
+    ap.add_argument("--jobs", type=int, default=256, help="mixed: jobs drawn from the seeded generator")
+    ap.add_argument("--tile-bits", type=int, default=0, help="mixed: log2 of the tile length (0 = engine default)")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="mixed: skip the plain-form comparison")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-sample-bits", type=int, default=None)
+    ap.add_argument("--max-factors", type=int, default=1024)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (--gpus {args.gpus} requested); the engine shards by prime "
+              "window across GPUs with no exchange in the hot loop, and multi-GPU runs are not part of this build.",
+              file=sys.stderr)
+        return 2
+
+    import torch
+
+    import cutrialdive_amd as ctd
+    from cutrialdive_amd import _native as nat
+
+    if args.workload == "mixed":
+        if args.steps is None:
+            args.steps = 1
+        if args.cpu_sample_bits is None:
+            args.cpu_sample_bits = 19
+        return run_mixed(args, torch, ctd, nat)
+    if args.cpu_sample_bits is None:
+        args.cpu_sample_bits = 29
+    return run_single(args, torch, ctd, nat)
 
 
 if __name__ == "__main__":

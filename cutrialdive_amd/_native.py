@@ -24,6 +24,9 @@ CTD_E_UNSUPPORTED = -4
 CTD_E_BUSY = -5
 
 CTD_RUN_GENERIC_ONLY = 1
+CTD_BATCH_HOST_BUFFERS = 1
+CTD_BATCH_NO_GRAPH = 2
+CTD_BATCH_NO_AUTOTUNE = 4
 
 
 class CtdJob(ctypes.Structure):
@@ -74,6 +77,24 @@ class CtdRunStats(ctypes.Structure):
     ]
 
 
+class CtdBatchOpts(ctypes.Structure):
+    _fields_ = [("tile_length", u64), ("streams", u32), ("flags", u32)]
+
+
+class CtdBatchStats(ctypes.Structure):
+    _fields_ = [
+        ("primes", u64),
+        ("updates", u64),
+        ("total_ms", ctypes.c_double),
+        ("build_ms", ctypes.c_double),
+        ("autotune_ms", ctypes.c_double),
+        ("tiles", u32),
+        ("graphs", u32),
+        ("tuned", u32),
+        ("reserved", u32),
+    ]
+
+
 # every symbol include/ctd_hip.h declares, with its ctypes signature
 SIGNATURES = {
     "ctd_hip_abi_version": (ctypes.c_int, []),
@@ -118,6 +139,23 @@ SIGNATURES = {
         [ctypes.c_void_p, ctypes.POINTER(CtdJob), ctypes.POINTER(u64), u64, ctypes.POINTER(u64), u32],
     ),
     "ctd_hip_valu_probe": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, u32, ctypes.POINTER(ctypes.c_double)]),
+    "ctd_hip_run_batch": (
+        ctypes.c_int,
+        [
+            ctypes.c_void_p,
+            ctypes.POINTER(CtdJob),
+            u32,
+            ctypes.POINTER(ctypes.c_void_p),
+            ctypes.POINTER(ctypes.c_void_p),
+            ctypes.POINTER(CtdBatchOpts),
+            ctypes.POINTER(CtdBatchStats),
+            ctypes.c_void_p,
+        ],
+    ),
+    "ctd_hip_batch_shape": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.POINTER(CtdJob), ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(u32)],
+    ),
 }
 
 PROBE_OPS = [

@@ -12,9 +12,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -26,11 +28,21 @@
 
 using namespace ctd;
 
+// When np_dev is given, the prime count is read from device memory (the
+// sieve of the same stream wrote it) and nprimes is the capacity bounding it.
+__device__ __forceinline__ uint64_t prime_count(uint64_t nprimes, const unsigned long long *np_dev)
+{
+    if (!np_dev) return nprimes;
+    const uint64_t c = *np_dev;
+    return c < nprimes ? c : nprimes;
+}
+
 // Generic kernel: any odd prime < 2^64, 64-bit Montgomery set-up.
 template <uint32_t FAM>
 __global__ void __launch_bounds__(256) tf_generic_kernel(JobArgs J, const uint64_t *__restrict__ primes,
-                                                          uint64_t nprimes)
+                                                          uint64_t nprimes, const unsigned long long *np_dev)
 {
+    nprimes = prime_count(nprimes, np_dev);
     const uint64_t tr = (nprimes + 63) & ~63ull; // wave-aligned prime slots per chunk
     const uint64_t work = tr * J.nchunks;
     for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < work;
@@ -57,8 +69,9 @@ __global__ void __launch_bounds__(256) tf_generic_kernel(JobArgs J, const uint64
 // Fast kernel: primes below 2^63 with the family's narrow step (ctd_tf_fast.hpp).
 template <uint32_t FAM>
 __global__ void __launch_bounds__(256) tf_fast_kernel(JobArgs J, const uint64_t *__restrict__ primes,
-                                                       uint64_t nprimes)
+                                                       uint64_t nprimes, const unsigned long long *np_dev)
 {
+    nprimes = prime_count(nprimes, np_dev);
     const uint64_t tr = (nprimes + 63) & ~63ull;
     const uint64_t work = tr * J.nchunks;
     for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < work;
@@ -94,6 +107,38 @@ __global__ void zero_u64_kernel(unsigned long long *p, uint32_t n)
     if (i < n) p[i] = 0;
 }
 
+// After a tile's sieve: keep its prime count and largest prime, and flag a
+// count above the slot's capacity (the sieve never writes past it).
+__global__ void tile_done_kernel(const unsigned long long *sc, uint64_t cap, unsigned long long *err,
+                                 unsigned long long *out)
+{
+    if (threadIdx.x == 0) {
+        const unsigned long long n = sc[0];
+        out[0] = n;
+        out[1] = sc[1];
+        if (n > cap) atomicOr(err, 1ull);
+    }
+}
+
+// Launch shape of a residue kernel: lanes per workgroup, waves per CU of a
+// grid-stride grid (0: one work item per lane) and the extra index-chunk split.
+struct LaunchShape {
+    uint32_t block = 256;
+    uint32_t wpc = 0;
+    uint32_t split = 1;
+};
+
+// One resident prime slot of the batched form, with its own stream.
+struct BatchSlot {
+    hipStream_t st = nullptr;
+    hipEvent_t ev = nullptr;
+    uint64_t *primes = nullptr;
+    uint64_t cap = 0;
+    uint32_t *segcnt = nullptr;
+    uint64_t segcap = 0;
+    unsigned long long *sc = nullptr; // [0] count, [1] largest prime
+};
+
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
@@ -120,6 +165,17 @@ struct ctd_hip_engine {
     unsigned long long *d_scalars = nullptr; // [0] total, [1] max prime
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     ctd_run_stats stats{};
+    // batched form
+    std::vector<BatchSlot> slots;
+    hipStream_t cap_stream = nullptr;
+    hipEvent_t ev_fork = nullptr;
+    unsigned long long *d_err = nullptr;
+    unsigned long long *d_tile = nullptr;
+    uint64_t tile_cap = 0;
+    uint32_t *d_scr_counts = nullptr;
+    uint64_t *d_scr_factors = nullptr;
+    uint64_t scr_cap = 0;
+    std::map<uint64_t, LaunchShape> shapes;
 };
 
 static std::string g_open_error;
@@ -314,29 +370,30 @@ bool valid_job(ctd_hip_engine *E, const ctd_job *j, int *code)
 }
 
 template <uint32_t FAM>
-void launch_tf(ctd_hip_engine *E, const JobArgs &J, uint64_t nprimes, bool fast, dim3 grid, dim3 block,
-               hipStream_t st)
+void launch_tf(const JobArgs &J, const uint64_t *primes, uint64_t nprimes, const unsigned long long *np_dev,
+               bool fast, dim3 grid, dim3 block, hipStream_t st)
 {
     if (fast)
-        hipLaunchKernelGGL(tf_fast_kernel<FAM>, grid, block, 0, st, J, E->d_primes, nprimes);
+        hipLaunchKernelGGL(tf_fast_kernel<FAM>, grid, block, 0, st, J, primes, nprimes, np_dev);
     else
-        hipLaunchKernelGGL(tf_generic_kernel<FAM>, grid, block, 0, st, J, E->d_primes, nprimes);
+        hipLaunchKernelGGL(tf_generic_kernel<FAM>, grid, block, 0, st, J, primes, nprimes, np_dev);
 }
 
-void launch_tf_any(ctd_hip_engine *E, const JobArgs &J, uint64_t nprimes, bool fast, dim3 grid, dim3 block,
-                   hipStream_t st)
+// nprimes is the count, or the capacity when np_dev holds the count on the device.
+void launch_tf_any(const JobArgs &J, const uint64_t *primes, uint64_t nprimes, const unsigned long long *np_dev,
+                   bool fast, dim3 grid, dim3 block, hipStream_t st)
 {
     switch (J.fam) {
-    case FAM_MERSENNE: launch_tf<FAM_MERSENNE>(E, J, nprimes, fast, grid, block, st); break;
-    case FAM_PROTH: launch_tf<FAM_PROTH>(E, J, nprimes, fast, grid, block, st); break;
-    case FAM_RIESEL: launch_tf<FAM_RIESEL>(E, J, nprimes, fast, grid, block, st); break;
-    case FAM_SMARANDACHE: launch_tf<FAM_SMARANDACHE>(E, J, nprimes, fast, grid, block, st); break;
-    case FAM_FIBONACCI: launch_tf<FAM_FIBONACCI>(E, J, nprimes, fast, grid, block, st); break;
-    case FAM_LUCAS: launch_tf<FAM_LUCAS>(E, J, nprimes, fast, grid, block, st); break;
-    case FAM_PROTH_FIXED_N: launch_tf<FAM_PROTH_FIXED_N>(E, J, nprimes, fast, grid, block, st); break;
-    case FAM_RIESEL_FIXED_N: launch_tf<FAM_RIESEL_FIXED_N>(E, J, nprimes, fast, grid, block, st); break;
-    case FAM_TRIBONACCI: launch_tf<FAM_TRIBONACCI>(E, J, nprimes, fast, grid, block, st); break;
-    default: launch_tf<FAM_PEANO>(E, J, nprimes, fast, grid, block, st); break;
+    case FAM_MERSENNE: launch_tf<FAM_MERSENNE>(J, primes, nprimes, np_dev, fast, grid, block, st); break;
+    case FAM_PROTH: launch_tf<FAM_PROTH>(J, primes, nprimes, np_dev, fast, grid, block, st); break;
+    case FAM_RIESEL: launch_tf<FAM_RIESEL>(J, primes, nprimes, np_dev, fast, grid, block, st); break;
+    case FAM_SMARANDACHE: launch_tf<FAM_SMARANDACHE>(J, primes, nprimes, np_dev, fast, grid, block, st); break;
+    case FAM_FIBONACCI: launch_tf<FAM_FIBONACCI>(J, primes, nprimes, np_dev, fast, grid, block, st); break;
+    case FAM_LUCAS: launch_tf<FAM_LUCAS>(J, primes, nprimes, np_dev, fast, grid, block, st); break;
+    case FAM_PROTH_FIXED_N: launch_tf<FAM_PROTH_FIXED_N>(J, primes, nprimes, np_dev, fast, grid, block, st); break;
+    case FAM_RIESEL_FIXED_N: launch_tf<FAM_RIESEL_FIXED_N>(J, primes, nprimes, np_dev, fast, grid, block, st); break;
+    case FAM_TRIBONACCI: launch_tf<FAM_TRIBONACCI>(J, primes, nprimes, np_dev, fast, grid, block, st); break;
+    default: launch_tf<FAM_PEANO>(J, primes, nprimes, np_dev, fast, grid, block, st); break;
     }
 }
 
@@ -451,7 +508,7 @@ int run_job(ctd_hip_engine *E, const ctd_job *job, const ctd_run_opts *opts, con
                 uint64_t grid = (opts && opts->grid_size > 0) ? (uint64_t)opts->grid_size : div_up(items, block);
                 grid = std::min<uint64_t>(grid, 1ull << 30);
                 HIPCHK(hipEventRecord(E->ev0, st));
-                launch_tf_any(E, J, np, fast, dim3((uint32_t)grid), dim3(block), st);
+                launch_tf_any(J, E->d_primes, np, nullptr, fast, dim3((uint32_t)grid), dim3(block), st);
                 HIPCHK(hipGetLastError());
                 HIPCHK(hipEventRecord(E->ev1, st));
                 HIPCHK(hipStreamSynchronize(st));
@@ -484,6 +541,8 @@ int run_job(ctd_hip_engine *E, const ctd_job *job, const ctd_run_opts *opts, con
     if (last_prime) *last_prime = lastp;
     return CTD_OK;
 }
+
+#include "ctd_batch.hpp"
 
 } // namespace
 
@@ -537,6 +596,7 @@ void ctd_hip_close(ctd_hip_engine *E)
     (void)hipFree(E->d_segcnt);
     (void)hipFree(E->d_segoff);
     (void)hipFree(E->d_scalars);
+    release_batch(E);
     if (E->ev0) (void)hipEventDestroy(E->ev0);
     if (E->ev1) (void)hipEventDestroy(E->ev1);
     if (E->stream) (void)hipStreamDestroy(E->stream);
@@ -589,6 +649,45 @@ int ctd_hip_sieve(ctd_hip_engine *E, uint64_t start, uint64_t end, uint64_t *out
         return fail_hip(E, h);
     }
     return CTD_OK;
+}
+
+int ctd_hip_run_batch(ctd_hip_engine *E, const ctd_job *jobs, uint32_t njobs, uint32_t *const *counts,
+                      uint64_t *const *factors, const ctd_batch_opts *opts, ctd_batch_stats *stats, void *stream)
+{
+    if (!E) return CTD_E_INVALID;
+    if (stats) *stats = ctd_batch_stats{};
+    if (!njobs) return CTD_OK;
+    if (!jobs) return fail(E, CTD_E_INVALID, "jobs is NULL");
+    try {
+        HIPCHK(hipSetDevice(E->device));
+        return run_batch(E, jobs, njobs, counts, factors, opts, stats, stream ? (hipStream_t)stream : E->stream);
+    } catch (const HipError &h) {
+        return fail_hip(E, h);
+    }
+}
+
+int ctd_hip_batch_shape(ctd_hip_engine *E, const ctd_job *job, uint32_t *block, uint32_t *waves_per_cu,
+                        uint32_t *chunk_split)
+{
+    if (!E || !job) return CTD_E_INVALID;
+    try {
+        HIPCHK(hipSetDevice(E->device));
+        // a plan needs factors buffers: the probe writes to scratch only
+        const uint64_t N = job->n1 > job->n0 ? job->n1 - job->n0 : 0;
+        ensure_scratch(E, std::max<uint64_t>(N, 1));
+        uint32_t *c = E->d_scr_counts;
+        uint64_t *f = E->d_scr_factors;
+        std::vector<JobPlan> plans;
+        uint32_t tuned = 0;
+        int rc = plan_jobs(E, job, 1, &c, &f, true, plans, &tuned);
+        if (rc != CTD_OK) return rc;
+        if (block) *block = plans[0].shape.block;
+        if (waves_per_cu) *waves_per_cu = plans[0].shape.wpc;
+        if (chunk_split) *chunk_split = plans[0].shape.split;
+        return CTD_OK;
+    } catch (const HipError &h) {
+        return fail_hip(E, h);
+    }
 }
 
 int ctd_hip_trial_factor_device(ctd_hip_engine *E, const ctd_job *job, const ctd_run_opts *opts,
@@ -677,7 +776,8 @@ int ctd_hip_residues(ctd_hip_engine *E, const ctd_job *job, const uint64_t *prim
         const bool fast = !(flags & CTD_RUN_GENERIC_ONLY) && Fast<0>::fits(pmax);
         J.flags = (J.fam == FAM_SMARANDACHE && sm_narrow_ok(J.param, J.n1, pmax)) ? JOB_SM_NARROW : 0u;
         const uint64_t items = (nprimes + 63) & ~63ull;
-        launch_tf_any(E, J, nprimes, fast, dim3((uint32_t)div_up(items, 256)), dim3(256), E->stream);
+        launch_tf_any(J, E->d_primes, nprimes, nullptr, fast, dim3((uint32_t)div_up(items, 256)), dim3(256),
+                      E->stream);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(out, dr, nprimes * sizeof(uint64_t), hipMemcpyDeviceToHost, E->stream));
         HIPCHK(hipStreamSynchronize(E->stream));

@@ -268,6 +268,44 @@ class Engine:
         self._check(self.lib.ctd_hip_last_stats(self.h, ctypes.byref(st)))
         return st
 
+    def run_batch_raw(self, jobs, counts_ptrs, factors_ptrs, flags: int = 0, tile_length: int = 0, streams: int = 0,
+                      stream: int = 0) -> nat.CtdBatchStats:
+        """ctd_hip_run_batch on buffer addresses (device pointers, or host arrays with
+        CTD_BATCH_HOST_BUFFERS)."""
+        n = len(jobs)
+        arr = (nat.CtdJob * max(n, 1))(*jobs)
+        cp = (ctypes.c_void_p * max(n, 1))(*[ctypes.c_void_p(int(p)) for p in counts_ptrs])
+        fp = (ctypes.c_void_p * max(n, 1))(*[ctypes.c_void_p(int(p)) for p in factors_ptrs])
+        opts = nat.CtdBatchOpts(tile_length, streams, flags)
+        st = nat.CtdBatchStats()
+        self._check(
+            self.lib.ctd_hip_run_batch(
+                self.h, arr, n, cp, fp, ctypes.byref(opts), ctypes.byref(st),
+                ctypes.c_void_p(stream) if stream else None,
+            )
+        )
+        return st
+
+    def run_batch(self, jobs, flags: int = 0, tile_length: int = 0, streams: int = 0):
+        """Batched form on host arrays: returns ([(counts, factors)] per job, stats)."""
+        bufs = []
+        for j in jobs:
+            N = max(j.n1 - j.n0, 0)
+            bufs.append((np.zeros(max(N, 1), dtype=np.uint32), np.zeros(max(N * j.max_factors, 1), dtype=np.uint64)))
+        st = self.run_batch_raw(
+            jobs, [c.ctypes.data for c, _ in bufs], [f.ctypes.data for _, f in bufs],
+            flags | nat.CTD_BATCH_HOST_BUFFERS, tile_length, streams,
+        )
+        out = [(c[: max(j.n1 - j.n0, 0)], f[: max(j.n1 - j.n0, 0) * j.max_factors]) for (c, f), j in zip(bufs, jobs)]
+        return out, st
+
+    def batch_shape(self, job: nat.CtdJob):
+        """(block, waves per CU, chunk split) the batch autotuner picks for the job."""
+        b, w, s = nat.u32(), nat.u32(), nat.u32()
+        self._check(self.lib.ctd_hip_batch_shape(self.h, ctypes.byref(job), ctypes.byref(b), ctypes.byref(w),
+                                                 ctypes.byref(s)))
+        return b.value, w.value, s.value
+
 
 def trial_factor(
     spec: NumSeqSpec,

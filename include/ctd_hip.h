@@ -139,6 +139,51 @@ typedef struct ctd_run_stats {
 } ctd_run_stats;
 int ctd_hip_last_stats(const ctd_hip_engine *eng, ctd_run_stats *out);
 
+/* ---- Batched form ------------------------------------------------------
+ * Many jobs of any family, index range and prime window in one hipGraph
+ * launch.  Each job's prime window is cut into tiles; a tile is sieved into
+ * one of `streams` resident prime slots and propagated over the job's
+ * indices by that family's residue kernel, while the other slots sieve and
+ * propagate other tiles.  The launch shape of each family (workgroup size,
+ * waves per CU, index-chunk split) is chosen by a timed probe on first use and
+ * cached in the engine.  No reference counterpart: the reference runs one job
+ * per process (trial_factoring.cuh:336-513 per job); results per job are the
+ * same as ctd_hip_trial_factor's. */
+#define CTD_BATCH_HOST_BUFFERS 1u /* counts/factors are host arrays (copied in and back) */
+#define CTD_BATCH_NO_GRAPH 2u     /* issue the tiles eagerly on the streams (no graph) */
+#define CTD_BATCH_NO_AUTOTUNE 4u  /* fixed shape: 256-lane groups, one item per lane, no extra chunking */
+
+typedef struct ctd_batch_opts {
+    uint64_t tile_length;  /* integers of a job's prime window per tile (0 = 2^28) */
+    uint32_t streams;      /* tiles in flight = prime slots (0 = 4, at most 16) */
+    uint32_t flags;        /* CTD_BATCH_* */
+} ctd_batch_opts;
+
+typedef struct ctd_batch_stats {
+    uint64_t primes;       /* odd primes tested over all jobs */
+    uint64_t updates;      /* sum over jobs of primes x indices */
+    double total_ms;       /* device time of the batch launch(es), HIP events */
+    double build_ms;       /* host time to capture and instantiate the graph(s) */
+    double autotune_ms;    /* host time spent in launch-shape probes during this call */
+    uint32_t tiles;
+    uint32_t graphs;       /* graph launches (tiles are split into graphs of <= 1024 tiles) */
+    uint32_t tuned;        /* launch shapes probed during this call */
+    uint32_t reserved;
+} ctd_batch_stats;
+
+/* counts[j] / factors[j]: job j's factors buffer (device pointers, or host
+ * arrays with CTD_BATCH_HOST_BUFFERS), accumulated into like
+ * ctd_hip_trial_factor.  stream is a hipStream_t (NULL = the engine's). */
+int ctd_hip_run_batch(ctd_hip_engine *eng, const ctd_job *jobs, uint32_t njobs,
+                      uint32_t *const *counts, uint64_t *const *factors,
+                      const ctd_batch_opts *opts, ctd_batch_stats *stats, void *stream);
+
+/* Launch shape chosen for a job by the batch autotuner (probing it if needed):
+ * block (lanes per workgroup), waves per CU of the grid (0 = one item per
+ * lane) and the index-chunk split.  Test and report hook. */
+int ctd_hip_batch_shape(ctd_hip_engine *eng, const ctd_job *job, uint32_t *block,
+                        uint32_t *waves_per_cu, uint32_t *chunk_split);
+
 /* Test hook of the residue kernels: for each odd modulus primes[i] (host
  * array), runs the same kernel path as ctd_hip_trial_factor over [n0, n1) and
  * writes the plain residue S(n1-1) mod primes[i] to out[i] (host array).

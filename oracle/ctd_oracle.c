@@ -769,6 +769,7 @@ int ctdo_trial_factor(const ctdo_job *job, uint32_t threads, uint32_t *counts, u
             if (!r) push_factor(&t, n + 1 - job->n0, 2);
         }
     }
+    if (f0 < 3) f0 = 3; /* no odd prime below 3: [0, 2) and [1, 2) hold no prime at all */
     if (f0 < job->f1) {
         if (threads < 1) threads = 1;
         pool_t pl = {&t, f0, job->f1, 1ull << 22, 0, sieve_base_make(f0, job->f1)};

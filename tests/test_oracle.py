@@ -296,6 +296,18 @@ def test_tf_matches_bruteforce_small():
             assert got[i] == want, (seq, param, n)
 
 
+@pytest.mark.parametrize("f0,f1", [(0, 2), (1, 2), (0, 1), (2, 3), (0, 3), (1, 8), (3, 4)])
+def test_tf_tiny_prime_windows_match_bruteforce(f0, f1):
+    # windows that hold no prime, only 2, only 3: no other number is ever listed
+    for seq, param in FAMS:
+        n0, n1 = (2, 30)
+        got = o.trial_factor(seq, param, n0, n1, f0, f1, 64)
+        primes = list(sympy.primerange(f0, f1))
+        for i, n in enumerate(range(n0, n1)):
+            v = exact_value(seq, param, n)
+            assert got[i] == [p for p in primes if v % p == 0], (seq, param, n, f0, f1)
+
+
 def test_tf_threads_equivalent():
     a = o.trial_factor(o.PROTH, 1279, 0, 300, 0, 1 << 20, threads=1)
     b = o.trial_factor(o.PROTH, 1279, 0, 300, 0, 1 << 20, threads=8)
