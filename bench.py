@@ -108,6 +108,8 @@ def run_mixed(args, torch, ctd, nat):
     from cutrialdive_amd import workloads
 
     jobs = workloads.mixed_jobs(args.jobs)
+    if args.family:
+        jobs = [j for j in jobs if j.kind == args.family]
     eng = ctd.Engine(0)
     name, mem, cu = eng.device_info()
     Ns = [j.n1 - j.n0 for j in jobs]
@@ -120,7 +122,7 @@ def run_mixed(args, torch, ctd, nat):
     cptr = [d_counts.data_ptr() + 4 * int(offs[k]) for k in range(len(jobs))]
     fptr = [d_factors.data_ptr() + 8 * mx * int(offs[k]) for k in range(len(jobs))]
     cjobs = [nat.CtdJob(j.seq_id, j.max_factors, j.param, j.n0, j.n1, j.f0, j.f1) for j in jobs]
-    flags = nat.CTD_BATCH_NO_AUTOTUNE if args.no_autotune else 0
+    flags = (nat.CTD_BATCH_NO_AUTOTUNE if args.no_autotune else 0) | (nat.CTD_BATCH_NO_GRAPH if args.no_graph else 0)
 
     # warm-up: launch-shape probes for every (family, kind, index class), then
     # a batch on the first 2^24 integers of every window
@@ -381,6 +383,8 @@ def main() -> int:
     ap.add_argument("--jobs", type=int, default=256, help="mixed: jobs drawn from the seeded generator")
     ap.add_argument("--tile-bits", type=int, default=0, help="mixed: log2 of the tile length (0 = engine default)")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="mixed: issue the tiles eagerly (profilers)")
+    ap.add_argument("--family", default=None, help="mixed: keep only the jobs of one family (per-family rates)")
     ap.add_argument("--no-plain", action="store_true", help="mixed: skip the plain-form comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-sample-bits", type=int, default=None)
@@ -402,7 +406,7 @@ def main() -> int:
         if args.steps is None:
             args.steps = 1
         if args.cpu_sample_bits is None:
-            args.cpu_sample_bits = 19
+            args.cpu_sample_bits = 16
         return run_mixed(args, torch, ctd, nat)
     if args.cpu_sample_bits is None:
         args.cpu_sample_bits = 29

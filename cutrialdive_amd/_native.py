@@ -169,6 +169,10 @@ PROBE_OPS = [
     "v_alignbit_b32",
     "v_min_u32",
     "v_lshlrev_b64",
+    "v_add_f64",
+    "v_fma_f64",
+    "v_mul_f64",
+    "v_max_f64",
 ]
 
 _lib = None

@@ -22,7 +22,11 @@ enum ProbeOp : int {
     PROBE_ALIGNBIT_B32 = 7,
     PROBE_MIN_U32 = 8,
     PROBE_LSHLREV_B64 = 9,
-    PROBE_COUNT = 10
+    PROBE_ADD_F64 = 10,
+    PROBE_FMA_F64 = 11,
+    PROBE_MUL_F64 = 12,
+    PROBE_MAX_F64 = 13,
+    PROBE_COUNT = 14
 };
 
 constexpr uint32_t PROBE_CHAINS = 8;
@@ -39,7 +43,31 @@ template <int OP>
 __global__ void __launch_bounds__(256) valu_probe_kernel(uint32_t *out, uint32_t iters)
 {
     const uint32_t b = (blockIdx.x * 2654435761u) | 1u;
-    if constexpr (OP == PROBE_MAD_U64_U32 || OP == PROBE_LSHL_ADD_U64 || OP == PROBE_LSHLREV_B64) {
+    if constexpr (OP >= PROBE_ADD_F64) {
+        // values stay finite: |a| grows by at most (1 + 2^-20) per step
+        const double c = 1.0 + 0x1p-20 * (double)(b & 7);
+        double a[PROBE_CHAINS];
+        for (uint32_t j = 0; j < PROBE_CHAINS; ++j) a[j] = 1.0 + threadIdx.x * 0x1p-12 + j;
+        for (uint32_t it = 0; it < iters; ++it) {
+#pragma unroll
+            for (int d = 0; d < (int)PROBE_DEPTH; ++d) {
+#pragma unroll
+                for (int j = 0; j < (int)PROBE_CHAINS; ++j) {
+                    if constexpr (OP == PROBE_ADD_F64)
+                        asm volatile("v_add_f64 %0, %0, %1" : "+v"(a[j]) : "v"(c));
+                    else if constexpr (OP == PROBE_FMA_F64)
+                        asm volatile("v_fma_f64 %0, %0, %1, %1" : "+v"(a[j]) : "v"(c));
+                    else if constexpr (OP == PROBE_MUL_F64)
+                        asm volatile("v_mul_f64 %0, %0, %1" : "+v"(a[j]) : "v"(c));
+                    else
+                        asm volatile("v_max_f64 %0, %0, %1" : "+v"(a[j]) : "v"(c));
+                }
+            }
+        }
+        double x = 0;
+        for (uint32_t j = 0; j < PROBE_CHAINS; ++j) x += a[j];
+        if (x == 1234567.0) out[0] = 1u;
+    } else if constexpr (OP == PROBE_MAD_U64_U32 || OP == PROBE_LSHL_ADD_U64 || OP == PROBE_LSHLREV_B64) {
         uint64_t a[PROBE_CHAINS];
         for (uint32_t j = 0; j < PROBE_CHAINS; ++j) a[j] = threadIdx.x * 0x9e3779b97f4a7c15ull + j;
         for (uint32_t it = 0; it < iters; ++it) {

@@ -198,7 +198,8 @@ int ctd_hip_residues(ctd_hip_engine *eng, const ctd_job *job, const uint64_t *pr
 /* VALU throughput probe: lane-instructions per second of one instruction
  * (op: 0 v_add_u32, 1 v_mul_lo_u32, 2 v_mul_hi_u32, 3 v_mad_u64_u32,
  * 4 v_mad_u32_u24, 5 v_lshl_add_u64, 6 v_cndmask_b32, 7 v_alignbit_b32,
- * 8 v_min_u32, 9 v_lshlrev_b64) over independent chains on every CU.  The
+ * 8 v_min_u32, 9 v_lshlrev_b64, 10 v_add_f64, 11 v_fma_f64, 12 v_mul_f64,
+ * 13 v_max_f64) over independent chains on every CU.  The
  * measured integer peak the roofline is quoted against.  No reference
  * counterpart: the reference publishes no device peak. */
 int ctd_hip_valu_probe(ctd_hip_engine *eng, int op, uint32_t iters, double *lane_ops_per_s);
