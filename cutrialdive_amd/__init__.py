@@ -10,6 +10,7 @@ from .engine import (  # noqa: F401
     NumSeqId,
     NumSeqSpec,
     PEANO_SEQ_ID,
+    Server,
     TrialFactoringOptions,
     num_seq_id_from_string,
     spec_param,
@@ -22,6 +23,7 @@ __all__ = [
     "NumSeqId",
     "NumSeqSpec",
     "PEANO_SEQ_ID",
+    "Server",
     "TrialFactoringOptions",
     "num_seq_id_from_string",
     "spec_param",

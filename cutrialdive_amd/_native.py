@@ -95,6 +95,26 @@ class CtdBatchStats(ctypes.Structure):
     ]
 
 
+class CtdServerOpts(ctypes.Structure):
+    _fields_ = [("memory_budget", u64), ("tile_length", u64), ("streams", u32), ("queue_limit", u32)]
+
+
+class CtdServerStats(ctypes.Structure):
+    _fields_ = [
+        ("submitted", u64),
+        ("admitted", u64),
+        ("completed", u64),
+        ("failed", u64),
+        ("tiles", u64),
+        ("primes", u64),
+        ("updates", u64),
+        ("active_max", u32),
+        ("pending_max", u32),
+        ("pending", u32),
+        ("reserved", u32),
+    ]
+
+
 # every symbol include/ctd_hip.h declares, with its ctypes signature
 SIGNATURES = {
     "ctd_hip_abi_version": (ctypes.c_int, []),
@@ -152,6 +172,19 @@ SIGNATURES = {
             ctypes.c_void_p,
         ],
     ),
+    "ctd_hip_server_open": (
+        ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(CtdServerOpts), ctypes.POINTER(ctypes.c_void_p)]
+    ),
+    "ctd_hip_server_submit": (
+        ctypes.c_int,
+        [ctypes.c_void_p, ctypes.POINTER(CtdJob), ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(u64)],
+    ),
+    "ctd_hip_server_poll": (
+        ctypes.c_int, [ctypes.c_void_p, u64, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)]
+    ),
+    "ctd_hip_server_wait": (ctypes.c_int, [ctypes.c_void_p, u64, ctypes.POINTER(ctypes.c_double)]),
+    "ctd_hip_server_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(CtdServerStats)]),
+    "ctd_hip_server_close": (ctypes.c_int, [ctypes.c_void_p]),
     "ctd_hip_batch_shape": (
         ctypes.c_int,
         [ctypes.c_void_p, ctypes.POINTER(CtdJob), ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(u32)],

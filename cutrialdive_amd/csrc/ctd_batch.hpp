@@ -157,6 +157,44 @@ void ensure_scratch(ctd_hip_engine *E, uint64_t n)
     E->scr_cap = n;
 }
 
+// The operations of one tile on its slot's stream: zero the slot counters,
+// sieve, keep the count (tile_out[0..1]), propagate.
+void issue_tile(ctd_hip_engine *E, const JobPlan &P, BatchSlot &S, uint64_t blo, uint64_t bhi, uint64_t start,
+                unsigned long long *tile_out, bool accumulate = false)
+{
+    hipStream_t st = S.st;
+    SieveArgs A{};
+    A.lo0 = blo;
+    A.start = start;
+    A.end = bhi;
+    A.sp = E->d_sp;
+    A.sp_mu = E->d_mu;
+    A.nsp = E->nsp;
+    A.nsmall = E->nsmall;
+    A.mode = SIEVE_DENSE;
+    A.seg_counts = S.segcnt;
+    A.out64 = S.primes;
+    A.total = S.sc;
+    A.maxp = S.sc + 1;
+    A.cap = S.cap;
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, st, S.sc, 2u);
+    hipLaunchKernelGGL(sieve_kernel, dim3((uint32_t)div_up(bhi - blo, SEG_SPAN)), dim3(SIEVE_THREADS), 0, st, A);
+    hipLaunchKernelGGL(tile_done_kernel, dim3(1), dim3(64), 0, st, S.sc, S.cap, E->d_err, tile_out,
+                       (uint32_t)accumulate);
+    const uint64_t est = (uint64_t)est_primes(start, bhi);
+    JobArgs J = tile_args(E, P, est, bhi - 1);
+    const bool fast = Fast<0>::fits(bhi - 1);
+    launch_tf_any(J, S.primes, S.cap, S.sc, fast, tile_grid(E, P.shape, est, J.nchunks), dim3(P.shape.block), st);
+    HIPCHK(hipGetLastError());
+}
+
+// Slot buffers able to hold a tile of tile_len integers.
+void ensure_slot_buffers(BatchSlot &s, uint64_t tile_len)
+{
+    ensure_buf(s.primes, s.cap, prime_cap(tile_len));
+    ensure_buf(s.segcnt, s.segcap, div_up(tile_len, SEG_SPAN) + 1);
+}
+
 // Times the candidate shapes on a probe tile of the job and returns the best.
 LaunchShape tune_shape(ctd_hip_engine *E, const JobPlan &P, bool fast)
 {
@@ -281,35 +319,11 @@ void issue_tiles(ctd_hip_engine *E, const std::vector<JobPlan> &plans, const std
         const Tile &T = tiles[t];
         const JobPlan &P = plans[T.job];
         BatchSlot &S = E->slots[T.stream];
-        hipStream_t st = S.st;
-        SieveArgs A{};
-        A.lo0 = T.blo;
-        A.start = T.start;
-        A.end = T.bhi;
-        A.sp = E->d_sp;
-        A.sp_mu = E->d_mu;
-        A.nsp = E->nsp;
-        A.nsmall = E->nsmall;
-        A.mode = SIEVE_DENSE;
-        A.seg_counts = S.segcnt;
-        A.out64 = S.primes;
-        A.total = S.sc;
-        A.maxp = S.sc + 1;
-        A.cap = S.cap;
-        hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, st, S.sc, 2u);
-        hipLaunchKernelGGL(sieve_kernel, dim3((uint32_t)div_up(T.bhi - T.blo, SEG_SPAN)), dim3(SIEVE_THREADS), 0,
-                           st, A);
-        hipLaunchKernelGGL(tile_done_kernel, dim3(1), dim3(64), 0, st, S.sc, S.cap, E->d_err, E->d_tile + 2 * t);
         if (T.first && P.has2) {
             uint32_t g = (uint32_t)std::min<uint64_t>(div_up(P.N, 256), 4096);
-            hipLaunchKernelGGL(mod2_kernel, dim3(g), dim3(256), 0, st, P.J);
+            hipLaunchKernelGGL(mod2_kernel, dim3(g), dim3(256), 0, S.st, P.J);
         }
-        const uint64_t est = (uint64_t)est_primes(T.start, T.bhi);
-        JobArgs J = tile_args(E, P, est, T.bhi - 1);
-        const bool fast = Fast<0>::fits(T.bhi - 1);
-        launch_tf_any(J, S.primes, S.cap, S.sc, fast, tile_grid(E, P.shape, est, J.nchunks), dim3(P.shape.block),
-                      st);
-        HIPCHK(hipGetLastError());
+        issue_tile(E, P, S, T.blo, T.bhi, T.start, E->d_tile + 2 * t);
     }
     for (uint32_t s = 0; s < nst; ++s) {
         HIPCHK(hipEventRecord(E->slots[s].ev, E->slots[s].st));

@@ -14,6 +14,11 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -107,15 +112,20 @@ __global__ void zero_u64_kernel(unsigned long long *p, uint32_t n)
     if (i < n) p[i] = 0;
 }
 
-// After a tile's sieve: keep its prime count and largest prime, and flag a
-// count above the slot's capacity (the sieve never writes past it).
+// After a tile's sieve: keep its prime count and largest prime (or add the
+// count to a running total), and flag a count above the slot's capacity (the
+// sieve never writes past it).
 __global__ void tile_done_kernel(const unsigned long long *sc, uint64_t cap, unsigned long long *err,
-                                 unsigned long long *out)
+                                 unsigned long long *out, uint32_t accumulate)
 {
     if (threadIdx.x == 0) {
         const unsigned long long n = sc[0];
-        out[0] = n;
-        out[1] = sc[1];
+        if (accumulate) {
+            atomicAdd(out, n); // tiles of one job run on several streams
+        } else {
+            out[0] = n;
+            out[1] = sc[1];
+        }
         if (n > cap) atomicOr(err, 1ull);
     }
 }
@@ -543,6 +553,7 @@ int run_job(ctd_hip_engine *E, const ctd_job *job, const ctd_run_opts *opts, con
 }
 
 #include "ctd_batch.hpp"
+#include "ctd_serve.hpp"
 
 } // namespace
 
@@ -688,6 +699,131 @@ int ctd_hip_batch_shape(ctd_hip_engine *E, const ctd_job *job, uint32_t *block, 
     } catch (const HipError &h) {
         return fail_hip(E, h);
     }
+}
+
+struct ctd_hip_server {
+    ctd_hip_server_impl impl;
+};
+
+int ctd_hip_server_open(ctd_hip_engine *E, const ctd_server_opts *opts, ctd_hip_server **out)
+{
+    if (!E || !out) return fail(E, CTD_E_INVALID, "invalid argument");
+    *out = nullptr;
+    ctd_hip_server *srv = new ctd_hip_server;
+    ctd_hip_server_impl *S = &srv->impl;
+    S->E = E;
+    if (opts) S->opts = *opts;
+    if (!S->opts.memory_budget) S->opts.memory_budget = SERVE_BUDGET_DEFAULT;
+    if (!S->opts.tile_length) S->opts.tile_length = SERVE_TILE_DEFAULT;
+    S->opts.tile_length = std::max<uint64_t>(div_up(S->opts.tile_length, SEG_SPAN), 1) * SEG_SPAN;
+    if (!S->opts.streams) S->opts.streams = 4;
+    S->opts.streams = std::min<uint32_t>(S->opts.streams, 16);
+    if (!S->opts.queue_limit) S->opts.queue_limit = SERVE_QUEUE_DEFAULT;
+    try {
+        HIPCHK(hipSetDevice(E->device));
+        if (!E->d_err) HIPCHK(hipMalloc((void **)&E->d_err, sizeof(unsigned long long)));
+        hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, E->stream, E->d_err, 1u);
+        HIPCHK(hipStreamSynchronize(E->stream));
+        HIPCHK(hipStreamCreateWithFlags(&S->copy_st, hipStreamNonBlocking));
+        S->slots.resize(S->opts.streams);
+        S->inflight.resize(S->opts.streams);
+        for (auto &sl : S->slots) {
+            HIPCHK(hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking));
+            HIPCHK(hipMalloc((void **)&sl.sc, 2 * sizeof(unsigned long long)));
+            ensure_slot_buffers(sl, S->opts.tile_length);
+        }
+        S->th = std::thread(serve_loop, S);
+    } catch (const HipError &h) {
+        release_server(S);
+        delete srv;
+        return fail_hip(E, h);
+    }
+    *out = srv;
+    return CTD_OK;
+}
+
+int ctd_hip_server_submit(ctd_hip_server *srv, const ctd_job *job, uint32_t *counts, uint64_t *factors,
+                          uint64_t *ticket)
+{
+    if (!srv || !job || !ticket) return CTD_E_INVALID;
+    ctd_hip_server_impl *S = &srv->impl;
+    int code;
+    if (!valid_job(S->E, job, &code)) return code;
+    const uint64_t N = job->n1 > job->n0 ? job->n1 - job->n0 : 0;
+    if (N && (!counts || !factors)) return fail(S->E, CTD_E_INVALID, "counts/factors are NULL");
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (S->fatal != CTD_OK) return fail(S->E, S->fatal, S->fatal_msg);
+    if (S->pending.size() >= S->opts.queue_limit) return fail(S->E, CTD_E_BUSY, "serving queue is full");
+    ServeJob *J = new ServeJob;
+    J->ticket = S->next_ticket++;
+    J->job = *job;
+    J->h_counts = counts;
+    J->h_factors = factors;
+    J->t_submit = std::chrono::steady_clock::now();
+    S->tickets[J->ticket] = Ticket{};
+    S->pending.push_back(J);
+    S->stats.submitted++;
+    S->stats.pending_max = std::max<uint32_t>(S->stats.pending_max, (uint32_t)S->pending.size());
+    *ticket = J->ticket;
+    S->cv_work.notify_one();
+    return CTD_OK;
+}
+
+int ctd_hip_server_poll(ctd_hip_server *srv, uint64_t ticket, int *state, double *latency_ms)
+{
+    if (!srv || !state) return CTD_E_INVALID;
+    ctd_hip_server_impl *S = &srv->impl;
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->tickets.find(ticket);
+    if (it == S->tickets.end()) return fail(S->E, CTD_E_INVALID, "unknown ticket");
+    *state = it->second.state;
+    if (latency_ms) *latency_ms = it->second.latency_ms;
+    return CTD_OK;
+}
+
+int ctd_hip_server_wait(ctd_hip_server *srv, uint64_t ticket, double *latency_ms)
+{
+    if (!srv) return CTD_E_INVALID;
+    ctd_hip_server_impl *S = &srv->impl;
+    std::unique_lock<std::mutex> lk(S->mu);
+    auto it = S->tickets.find(ticket);
+    if (it == S->tickets.end()) return fail(S->E, CTD_E_INVALID, "unknown ticket");
+    S->cv_done.wait(lk, [&] { return it->second.state == 2 || it->second.state < 0; });
+    if (latency_ms) *latency_ms = it->second.latency_ms;
+    if (it->second.state < 0) return fail(S->E, it->second.state, S->fatal_msg);
+    return CTD_OK;
+}
+
+int ctd_hip_server_stats(ctd_hip_server *srv, ctd_server_stats *out)
+{
+    if (!srv || !out) return CTD_E_INVALID;
+    std::lock_guard<std::mutex> lk(srv->impl.mu);
+    *out = srv->impl.stats;
+    out->pending = (uint32_t)srv->impl.pending.size();
+    return CTD_OK;
+}
+
+int ctd_hip_server_close(ctd_hip_server *srv)
+{
+    if (!srv) return CTD_E_INVALID;
+    ctd_hip_server_impl *S = &srv->impl;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->stop = true;
+        S->cv_work.notify_all();
+    }
+    if (S->th.joinable()) S->th.join();
+    int rc = S->fatal;
+    if (rc == CTD_OK) {
+        unsigned long long err = 0;
+        if (hipMemcpy(&err, S->E->d_err, sizeof err, hipMemcpyDeviceToHost) != hipSuccess || err)
+            rc = fail(S->E, CTD_E_DEVICE, "prime slot overflow in a served tile");
+    } else {
+        S->E->err = S->fatal_msg;
+    }
+    release_server(S);
+    delete srv;
+    return rc;
 }
 
 int ctd_hip_trial_factor_device(ctd_hip_engine *E, const ctd_job *job, const ctd_run_opts *opts,

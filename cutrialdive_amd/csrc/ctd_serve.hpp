@@ -1,0 +1,331 @@
+// ctd_serve.hpp -- serving form: continuous batching of jobs as they arrive.
+//
+// Included by ctd_engine.hip inside its internal namespace.  A server owns a
+// scheduler thread, S streams with one resident prime slot each, and the
+// engine's resident sieving base.  Jobs are submitted with caller-owned host
+// factors buffers and come back through poll/wait.
+//
+// Scheduler loop (one host thread, no device-side waits):
+//   admit   pending jobs while their factors buffers fit in the device-memory
+//           budget (the first job is always admitted): allocate the device
+//           buffers, copy the host buffers in, run the mod-2 kernel, and cut
+//           the prime window into tiles;
+//   issue   each stream keeps at most two tiles in flight; a free place goes
+//           to the oldest admitted job that still has tiles, so one job can
+//           spread over every stream and jobs that arrive later join as soon
+//           as the earlier ones have issued all their tiles;
+//   retire  a tile is done when its event has completed; a job whose tiles are
+//           all done is copied back to its host buffers on the copy stream and
+//           completes when that copy has, which frees its device memory.
+// Latency is measured from submit to the moment the scheduler sees the copy
+// complete.  Residues, hits and factor buffers are those of the plain form:
+// the same kernels, the same tiles, only the interleaving differs.
+
+constexpr uint64_t SERVE_TILE_DEFAULT = 1ull << 26;
+constexpr uint64_t SERVE_BUDGET_DEFAULT = 8ull << 30;
+constexpr uint32_t SERVE_QUEUE_DEFAULT = 65536;
+constexpr uint32_t SERVE_DEPTH = 2; // tiles in flight per stream
+
+struct ServeJob {
+    uint64_t ticket = 0;
+    ctd_job job{};
+    uint32_t *h_counts = nullptr;
+    uint64_t *h_factors = nullptr;
+    uint32_t *d_counts = nullptr;
+    uint64_t *d_factors = nullptr;
+    unsigned long long *d_acc = nullptr; // odd primes tested, summed over the tiles
+    unsigned long long h_acc[2] = {0, 0};
+    uint64_t bytes = 0;
+    JobPlan plan;
+    std::deque<Tile> tiles;  // not yet issued
+    uint32_t outstanding = 0; // issued, not yet retired
+    hipEvent_t ready = nullptr;
+    hipEvent_t done = nullptr;
+    bool copying = false;
+    std::chrono::steady_clock::time_point t_submit;
+};
+
+struct Ticket {
+    int state = 0; // 0 queued, 1 running, 2 done, < 0 error code
+    double latency_ms = 0;
+};
+
+struct InFlight {
+    hipEvent_t ev;
+    ServeJob *job;
+};
+
+struct ctd_hip_server_impl {
+    ctd_hip_engine *E = nullptr;
+    ctd_server_opts opts{};
+    std::vector<BatchSlot> slots;
+    std::vector<std::deque<InFlight>> inflight;
+    hipStream_t copy_st = nullptr;
+    std::vector<hipEvent_t> event_pool;
+
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<ServeJob *> pending;
+    std::unordered_map<uint64_t, Ticket> tickets;
+    uint64_t next_ticket = 1;
+    bool stop = false;
+    int fatal = CTD_OK;
+    std::string fatal_msg;
+    ctd_server_stats stats{};
+
+    std::deque<ServeJob *> active; // scheduler thread only
+    uint64_t used_bytes = 0;
+    std::thread th;
+};
+
+hipEvent_t pool_event(ctd_hip_server_impl *S)
+{
+    if (!S->event_pool.empty()) {
+        hipEvent_t e = S->event_pool.back();
+        S->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    return e;
+}
+
+void finish_ticket(ctd_hip_server_impl *S, ServeJob *J, int state)
+{
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - J->t_submit).count();
+    std::lock_guard<std::mutex> lk(S->mu);
+    Ticket &t = S->tickets[J->ticket];
+    t.state = state;
+    t.latency_ms = ms;
+    if (state == 2) {
+        S->stats.completed++;
+        S->stats.primes += J->h_acc[0];
+        S->stats.updates += J->h_acc[0] * (J->job.n1 > J->job.n0 ? J->job.n1 - J->job.n0 : 0);
+    } else {
+        S->stats.failed++;
+    }
+    S->cv_done.notify_all();
+}
+
+// Admission of one job (scheduler thread).  Returns false when the job has to
+// wait for memory.
+bool admit(ctd_hip_server_impl *S, ServeJob *J)
+{
+    ctd_hip_engine *E = S->E;
+    const ctd_job &q = J->job;
+    const uint64_t N = q.n1 > q.n0 ? q.n1 - q.n0 : 0;
+    J->bytes = N * (4 + 8ull * q.max_factors);
+    if (!S->active.empty() && S->used_bytes + J->bytes > S->opts.memory_budget) return false;
+    if (!N || q.f1 <= q.f0) { // trial_factoring_impl.hpp:380-387: nothing to do
+        finish_ticket(S, J, 2);
+        delete J;
+        return true;
+    }
+    const uint64_t f0 = std::max<uint64_t>(q.f0, 3);
+    if (f0 < q.f1 && isqrt_u64(q.f1 - 1) > E->base_lim && E->d_sp) {
+        // the sieving base grows: no tile may be reading the old one
+        for (auto &sl : S->slots) HIPCHK(hipStreamSynchronize(sl.st));
+    }
+    HIPCHK(hipMalloc((void **)&J->d_counts, N * sizeof(uint32_t)));
+    HIPCHK(hipMalloc((void **)&J->d_factors, N * (uint64_t)q.max_factors * sizeof(uint64_t)));
+    HIPCHK(hipMalloc((void **)&J->d_acc, 2 * sizeof(unsigned long long)));
+    S->used_bytes += J->bytes;
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, S->copy_st, J->d_acc, 2u);
+    HIPCHK(hipMemcpyAsync(J->d_counts, J->h_counts, N * sizeof(uint32_t), hipMemcpyHostToDevice, S->copy_st));
+    HIPCHK(hipMemcpyAsync(J->d_factors, J->h_factors, N * (uint64_t)q.max_factors * sizeof(uint64_t),
+                          hipMemcpyHostToDevice, S->copy_st));
+    std::vector<JobPlan> plans;
+    uint32_t tuned = 0;
+    int rc = plan_jobs(E, &q, 1, &J->d_counts, &J->d_factors, false, plans, &tuned);
+    if (rc != CTD_OK) throw HipError{hipErrorInvalidValue, "job plan"};
+    J->plan = plans[0];
+    if (J->plan.f0 < J->plan.f1) {
+        const bool fast = Fast<0>::fits(J->plan.f1 - 1);
+        const bool narrow = q.seq_id == FAM_SMARANDACHE && sm_narrow_ok(q.param, q.n1, J->plan.f1 - 1);
+        auto it = E->shapes.find(shape_key(q.seq_id, fast, narrow, N));
+        if (it != E->shapes.end()) J->plan.shape = it->second;
+    }
+    if (J->plan.has2) {
+        uint32_t g = (uint32_t)std::min<uint64_t>(div_up(N, 256), 4096);
+        hipLaunchKernelGGL(mod2_kernel, dim3(g), dim3(256), 0, S->copy_st, J->plan.J);
+        HIPCHK(hipGetLastError());
+    }
+    J->ready = pool_event(S);
+    HIPCHK(hipEventRecord(J->ready, S->copy_st));
+    const uint64_t tl = S->opts.tile_length;
+    if (J->plan.f0 < J->plan.f1) {
+        const uint64_t lo0 = J->plan.f0 & ~1ull;
+        for (uint64_t blo = lo0; blo < J->plan.f1; blo += tl) {
+            const uint64_t bhi = (J->plan.f1 - blo > tl) ? blo + tl : J->plan.f1;
+            J->tiles.push_back(Tile{0, blo, bhi, std::max(blo, J->plan.f0), 0.0, 0u, false});
+            if (bhi == J->plan.f1) break;
+        }
+    }
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->tickets[J->ticket].state = 1;
+        S->stats.admitted++;
+        S->stats.active_max = std::max<uint32_t>(S->stats.active_max, (uint32_t)S->active.size() + 1);
+    }
+    S->active.push_back(J);
+    return true;
+}
+
+// One pass of the scheduler; returns true when it made progress.
+bool schedule_once(ctd_hip_server_impl *S)
+{
+    ctd_hip_engine *E = S->E;
+    bool progress = false;
+    // admit
+    for (;;) {
+        ServeJob *J = nullptr;
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            if (S->pending.empty()) break;
+            J = S->pending.front();
+        }
+        if (!admit(S, J)) break;
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->pending.pop_front();
+        progress = true;
+    }
+    // retire
+    for (size_t s = 0; s < S->slots.size(); ++s) {
+        auto &q = S->inflight[s];
+        while (!q.empty()) {
+            hipError_t e = hipEventQuery(q.front().ev);
+            if (e == hipErrorNotReady) break;
+            if (e != hipSuccess) throw HipError{e, "tile event"};
+            S->event_pool.push_back(q.front().ev);
+            q.front().job->outstanding--;
+            q.pop_front();
+            progress = true;
+        }
+    }
+    // issue
+    for (size_t s = 0; s < S->slots.size(); ++s) {
+        while (S->inflight[s].size() < SERVE_DEPTH) {
+            ServeJob *J = nullptr;
+            for (ServeJob *a : S->active)
+                if (!a->tiles.empty()) {
+                    J = a;
+                    break;
+                }
+            if (!J) break;
+            Tile T = J->tiles.front();
+            J->tiles.pop_front();
+            BatchSlot &sl = S->slots[s];
+            HIPCHK(hipStreamWaitEvent(sl.st, J->ready, 0));
+            issue_tile(E, J->plan, sl, T.blo, T.bhi, T.start, J->d_acc, true);
+            hipEvent_t ev = pool_event(S);
+            HIPCHK(hipEventRecord(ev, sl.st));
+            S->inflight[s].push_back(InFlight{ev, J});
+            J->outstanding++;
+            {
+                std::lock_guard<std::mutex> lk(S->mu);
+                S->stats.tiles++;
+            }
+            progress = true;
+        }
+    }
+    // complete
+    for (auto it = S->active.begin(); it != S->active.end();) {
+        ServeJob *J = *it;
+        if (!J->tiles.empty() || J->outstanding) {
+            ++it;
+            continue;
+        }
+        const uint64_t N = J->job.n1 - J->job.n0;
+        if (!J->copying) {
+            // every tile event has completed, so the device buffers are final
+            HIPCHK(hipMemcpyAsync(J->h_counts, J->d_counts, N * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                  S->copy_st));
+            HIPCHK(hipMemcpyAsync(J->h_factors, J->d_factors, N * (uint64_t)J->job.max_factors * sizeof(uint64_t),
+                                  hipMemcpyDeviceToHost, S->copy_st));
+            HIPCHK(hipMemcpyAsync(J->h_acc, J->d_acc, sizeof J->h_acc, hipMemcpyDeviceToHost, S->copy_st));
+            J->done = pool_event(S);
+            HIPCHK(hipEventRecord(J->done, S->copy_st));
+            J->copying = true;
+            progress = true;
+        }
+        hipError_t e = hipEventQuery(J->done);
+        if (e == hipErrorNotReady) {
+            ++it;
+            continue;
+        }
+        if (e != hipSuccess) throw HipError{e, "copy event"};
+        HIPCHK(hipFree(J->d_counts));
+        HIPCHK(hipFree(J->d_factors));
+        HIPCHK(hipFree(J->d_acc));
+        S->used_bytes -= J->bytes;
+        S->event_pool.push_back(J->ready);
+        S->event_pool.push_back(J->done);
+        finish_ticket(S, J, 2);
+        delete J;
+        it = S->active.erase(it);
+        progress = true;
+    }
+    (void)E;
+    return progress;
+}
+
+void fail_all(ctd_hip_server_impl *S, int code, const std::string &msg)
+{
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->fatal = code;
+    S->fatal_msg = msg;
+    for (ServeJob *J : S->active) {
+        S->tickets[J->ticket].state = code;
+        S->stats.failed++;
+        delete J;
+    }
+    S->active.clear();
+    for (ServeJob *J : S->pending) {
+        S->tickets[J->ticket].state = code;
+        S->stats.failed++;
+        delete J;
+    }
+    S->pending.clear();
+    S->cv_done.notify_all();
+}
+
+void serve_loop(ctd_hip_server_impl *S)
+{
+    try {
+        HIPCHK(hipSetDevice(S->E->device));
+        for (;;) {
+            bool progress = schedule_once(S);
+            std::unique_lock<std::mutex> lk(S->mu);
+            if (S->stop && S->pending.empty() && S->active.empty()) break;
+            if (progress) continue;
+            if (S->active.empty() && S->pending.empty()) {
+                S->cv_work.wait(lk, [&] { return S->stop || !S->pending.empty(); });
+            } else {
+                // device work in flight: poll its events again shortly
+                S->cv_work.wait_for(lk, std::chrono::microseconds(50));
+            }
+        }
+    } catch (const HipError &h) {
+        fail_all(S, h.code == hipErrorOutOfMemory ? CTD_E_NOMEM : CTD_E_DEVICE,
+                 std::string(h.what) + ": " + hipGetErrorString(h.code));
+    }
+}
+
+void release_server(ctd_hip_server_impl *S)
+{
+    for (auto &sl : S->slots) {
+        if (sl.st) (void)hipStreamSynchronize(sl.st);
+        (void)hipFree(sl.primes);
+        (void)hipFree(sl.segcnt);
+        (void)hipFree(sl.sc);
+        if (sl.ev) (void)hipEventDestroy(sl.ev);
+        if (sl.st) (void)hipStreamDestroy(sl.st);
+    }
+    for (auto &q : S->inflight)
+        for (auto &f : q) (void)hipEventDestroy(f.ev);
+    if (S->copy_st) {
+        (void)hipStreamSynchronize(S->copy_st);
+        (void)hipStreamDestroy(S->copy_st);
+    }
+    for (auto e : S->event_pool) (void)hipEventDestroy(e);
+}

@@ -334,3 +334,68 @@ def trial_factor(
         if own:
             eng.close()
     return FactoringResults.from_buffer(opts.n0, counts, factors, opts.max_factors_per_number)
+
+
+class Server:
+    """Serving form (ctd_hip_server_*): submit jobs while others run.
+
+    Each submitted job gets host numpy buffers in the factors_buffer layout;
+    wait() returns them with the job's latency.  While the server is open the
+    engine handle must not be used for other calls."""
+
+    def __init__(self, engine: Engine, memory_budget: int = 0, tile_length: int = 0, streams: int = 0,
+                 queue_limit: int = 0):
+        self.eng = engine
+        self.lib = engine.lib
+        opts = nat.CtdServerOpts(memory_budget, tile_length, streams, queue_limit)
+        h = ctypes.c_void_p()
+        engine._check(self.lib.ctd_hip_server_open(engine.h, ctypes.byref(opts), ctypes.byref(h)))
+        self.h = h
+        self.jobs = {}
+
+    def submit(self, job: nat.CtdJob, counts: Optional[np.ndarray] = None,
+               factors: Optional[np.ndarray] = None) -> int:
+        N = max(job.n1 - job.n0, 0)
+        if counts is None:
+            counts = np.zeros(max(N, 1), dtype=np.uint32)
+        if factors is None:
+            factors = np.zeros(max(N * job.max_factors, 1), dtype=np.uint64)
+        t = nat.u64()
+        rc = self.lib.ctd_hip_server_submit(self.h, ctypes.byref(job), counts.ctypes.data_as(ctypes.c_void_p),
+                                            factors.ctypes.data_as(ctypes.c_void_p), ctypes.byref(t))
+        if rc == nat.CTD_E_BUSY:
+            raise BlockingIOError(self.lib.ctd_hip_last_error(self.eng.h).decode())
+        self.eng._check(rc)
+        self.jobs[t.value] = (job, counts, factors)
+        return t.value
+
+    def poll(self, ticket: int):
+        st = ctypes.c_int()
+        lat = ctypes.c_double()
+        self.eng._check(self.lib.ctd_hip_server_poll(self.h, ticket, ctypes.byref(st), ctypes.byref(lat)))
+        return st.value, lat.value
+
+    def wait(self, ticket: int):
+        """(counts, factors, latency_ms) of a finished job; its buffers are released here."""
+        lat = ctypes.c_double()
+        self.eng._check(self.lib.ctd_hip_server_wait(self.h, ticket, ctypes.byref(lat)))
+        job, c, f = self.jobs.pop(ticket)
+        N = max(job.n1 - job.n0, 0)
+        return c[:N], f[:N * job.max_factors], lat.value
+
+    def stats(self) -> nat.CtdServerStats:
+        st = nat.CtdServerStats()
+        self.eng._check(self.lib.ctd_hip_server_stats(self.h, ctypes.byref(st)))
+        return st
+
+    def close(self) -> None:
+        if self.h:
+            rc = self.lib.ctd_hip_server_close(self.h)
+            self.h = None
+            self.eng._check(rc)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

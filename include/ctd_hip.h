@@ -184,6 +184,50 @@ int ctd_hip_run_batch(ctd_hip_engine *eng, const ctd_job *jobs, uint32_t njobs,
 int ctd_hip_batch_shape(ctd_hip_engine *eng, const ctd_job *job, uint32_t *block,
                         uint32_t *waves_per_cu, uint32_t *chunk_split);
 
+/* ---- Serving form -------------------------------------------------------
+ * Continuous batching: jobs are submitted while others run.  A scheduler
+ * thread admits them while their factors buffers fit in a device-memory
+ * budget, cuts their prime windows into tiles and interleaves the tiles of the
+ * admitted jobs on `streams` streams, each with a resident prime slot; the
+ * sieving base stays resident.  Each job's factors buffer is the one
+ * ctd_hip_trial_factor would produce.  No reference counterpart: the reference
+ * CLI runs one job per process.  While a server is open, the engine handle
+ * must not be used for other calls. */
+typedef struct ctd_hip_server ctd_hip_server;
+
+typedef struct ctd_server_opts {
+    uint64_t memory_budget; /* device bytes for admitted jobs' factors buffers (0 = 8 GiB) */
+    uint64_t tile_length;   /* integers of a prime window per tile (0 = 2^26) */
+    uint32_t streams;       /* streams = resident prime slots (0 = 4, at most 16) */
+    uint32_t queue_limit;   /* jobs waiting for admission before submit returns CTD_E_BUSY (0 = 65536) */
+} ctd_server_opts;
+
+typedef struct ctd_server_stats {
+    uint64_t submitted, admitted, completed, failed;
+    uint64_t tiles;         /* tiles issued */
+    uint64_t primes;        /* odd primes tested by completed jobs */
+    uint64_t updates;       /* primes x indices of completed jobs */
+    uint32_t active_max;    /* largest number of admitted jobs at once */
+    uint32_t pending_max;   /* largest queue of jobs waiting for admission */
+    uint32_t pending;       /* jobs waiting for admission now */
+    uint32_t reserved;
+} ctd_server_stats;
+
+int ctd_hip_server_open(ctd_hip_engine *eng, const ctd_server_opts *opts, ctd_hip_server **out);
+/* counts/factors: the job's host factors buffer (read at admission, written at
+ * completion); it must stay valid until the job is done.  Returns CTD_E_BUSY
+ * when queue_limit jobs are already waiting. */
+int ctd_hip_server_submit(ctd_hip_server *srv, const ctd_job *job, uint32_t *counts,
+                          uint64_t *factors, uint64_t *ticket);
+/* *state: 0 queued, 1 admitted, 2 done, negative: the CTD_E_* code of a failure.
+ * *latency_ms: submit to completion, once done. */
+int ctd_hip_server_poll(ctd_hip_server *srv, uint64_t ticket, int *state, double *latency_ms);
+/* Blocks until the job is done (or failed). */
+int ctd_hip_server_wait(ctd_hip_server *srv, uint64_t ticket, double *latency_ms);
+int ctd_hip_server_stats(ctd_hip_server *srv, ctd_server_stats *out);
+/* Completes every submitted job, then releases the server. */
+int ctd_hip_server_close(ctd_hip_server *srv);
+
 /* Test hook of the residue kernels: for each odd modulus primes[i] (host
  * array), runs the same kernel path as ctd_hip_trial_factor over [n0, n1) and
  * writes the plain residue S(n1-1) mod primes[i] to out[i] (host array).
