@@ -17,6 +17,13 @@ device batch of 2^31 integers of the prime window:
   cfg3a  Sm16, n in [1, 2000), primes in [2^40, 2^40 + 2^36)
   cfg3b  Proth k=3, n in [1, 10^5), primes in [2^40, 2^40 + 2^32)
 
+Serving workload (--workload serve): a Poisson stream of jobs drawn from the
+same generator, with 2^24-wide prime windows so that 10^4 jobs take about a
+minute, submitted to the serving form (ctd_hip_server_*) at offered loads of
+0.5, 0.7 and 0.9 of its measured capacity.  Reports p50/p99 job latency and
+sustained updates/s per load; every served job's factors buffer is compared
+with the plain form's.
+
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -261,6 +268,140 @@ def run_mixed(args, torch, ctd, nat):
     return 0
 
 
+def buffer_digest(c, f, mx):
+    """Order-free digest of a factors buffer (stored factors sorted per index)."""
+    import hashlib
+
+    h = hashlib.sha1(np.ascontiguousarray(c).tobytes())
+    h.update(np.sort(np.asarray(f).reshape(len(c), mx), axis=1).tobytes())
+    return h.hexdigest()
+
+
+def run_serve(args, torch, ctd, nat):
+    import threading
+
+    from cutrialdive_amd import workloads
+
+    specs = workloads.mixed_jobs(args.serve_jobs, seed=workloads.SEED + 2, window_bits=24)
+    cjobs = [nat.CtdJob(j.seq_id, j.max_factors, j.param, j.n0, j.n1, j.f0, j.f1) for j in specs]
+    mx = specs[0].max_factors
+    loads = [float(x) for x in args.loads.split(",")]
+    eng = ctd.Engine(0)
+    name, mem, cu = eng.device_info()
+    print(f"# device {name}, {cu} CUs; serving {len(specs)} jobs per load, loads {loads}", file=sys.stderr,
+          flush=True)
+    # launch shapes for every (family, kind, index class) of the stream, shared with the server
+    t0 = time.perf_counter()
+    seen = {}
+    for j, c in zip(specs, cjobs):
+        seen.setdefault((j.seq_id, j.n1 - j.n0 < 1024, j.n1 - j.n0 < 16384), c)
+    for c in seen.values():
+        eng.batch_shape(c)
+    print(f"# autotune {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+
+    def serve(jobs, arrivals):
+        """Submits jobs at the arrival times (None: all at once); returns
+        (latencies ms, digests, seconds from first submit to last completion, stats)."""
+        srv = ctd.Server(eng, tile_length=1 << 24, streams=args.serve_streams, fresh_buffers=True)
+        lat = [0.0] * len(jobs)
+        dig = [None] * len(jobs)
+        tickets = [None] * len(jobs)
+        ready = threading.Semaphore(0)
+
+        def consume():
+            for k in range(len(jobs)):
+                ready.acquire()
+                c, f, ms = srv.wait(tickets[k])
+                lat[k] = ms
+                dig[k] = buffer_digest(c, f, mx)
+
+        th = threading.Thread(target=consume)
+        th.start()
+        t_start = time.perf_counter()
+        for k, c in enumerate(jobs):
+            if arrivals is not None:
+                dt = t_start + arrivals[k] - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+            tickets[k] = srv.submit(c)
+            ready.release()
+        th.join()
+        elapsed = time.perf_counter() - t_start
+        st = srv.stats()
+        srv.close()
+        return lat, dig, elapsed, st
+
+    # capacity: the first jobs of the stream, all submitted at once
+    ncap = min(len(cjobs), args.capacity_jobs)
+    _, _, cap_s, cap_st = serve(cjobs[:ncap], None)
+    capacity = ncap / cap_s
+    print(f"# capacity {capacity:.1f} jobs/s ({cap_st.updates / cap_s:.3e} updates/s)", file=sys.stderr, flush=True)
+    # the same jobs in the batched form (one graph), for comparison
+    t0 = time.perf_counter()
+    _, bst = eng.run_batch(cjobs[:ncap], tile_length=1 << 24)
+    batch_s = time.perf_counter() - t0
+    print(f"# batched form on the same jobs: {ncap / batch_s:.1f} jobs/s ({bst.updates / batch_s:.3e} updates/s)",
+          file=sys.stderr, flush=True)
+
+    runs = []
+    for L in loads:
+        arr = workloads.poisson_arrivals(len(cjobs), L * capacity, seed=workloads.SEED + int(L * 1000))
+        lat, dig, el, st = serve(cjobs, arr)
+        lat_np = np.array(lat)
+        runs.append({"load": L, "offered_jobs_per_s": L * capacity, "jobs": len(cjobs), "seconds": el,
+                     "p50_ms": float(np.percentile(lat_np, 50)), "p99_ms": float(np.percentile(lat_np, 99)),
+                     "mean_ms": float(lat_np.mean()), "updates_per_s": st.updates / el, "updates": st.updates,
+                     "tiles": st.tiles, "active_max": st.active_max, "pending_max": st.pending_max,
+                     "digests": dig})
+        print(f"# load {L}: {el:.1f} s, p50 {runs[-1]['p50_ms']:.1f} ms, p99 {runs[-1]['p99_ms']:.1f} ms, "
+              f"{st.updates / el:.3e} updates/s, active max {st.active_max}", file=sys.stderr, flush=True)
+
+    # plain form, one job per call: the reference the served buffers are compared with
+    t0 = time.perf_counter()
+    plain = []
+    for c in cjobs:
+        N = c.n1 - c.n0
+        pc, pf, _ = eng.run_job(c.seq_id, c.param, c.n0, c.n1, c.f0, c.f1, mx)
+        plain.append(buffer_digest(pc, pf, mx))
+    plain_s = time.perf_counter() - t0
+    for r in runs:
+        r["factors_identical_to_plain"] = r.pop("digests") == plain
+    print(f"# plain form check {plain_s:.1f} s", file=sys.stderr, flush=True)
+
+    top = runs[-1]
+    result = {
+        "metric": METRIC,
+        "value": top["updates_per_s"],
+        "unit": "updates/s",
+        "n_gpus": 1,
+        "steps": len(loads),
+        "warmup": 0,
+        "ms_per_step": top["seconds"] * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic (seeded job generator and Poisson arrivals)",
+        "config": {
+            "workload": f"serving: Poisson stream of {len(cjobs)} jobs of the mixed-job distribution with 2^24-wide "
+                        f"prime windows, offered load {loads} of measured capacity; value = sustained updates/s "
+                        f"at load {top['load']}",
+            "capacity_jobs_per_s": capacity,
+            "capacity_updates_per_s": cap_st.updates / cap_s,
+            "capacity_sample_jobs": ncap,
+            "batched_form_same_jobs_per_s": ncap / batch_s,
+            "batched_form_same_updates_per_s": bst.updates / batch_s,
+            "tile_length": 1 << 24,
+            "streams": args.serve_streams,
+        },
+        "serving": runs,
+        "plain_check_seconds": plain_s,
+    }
+    print(json.dumps(result), flush=True)
+    eng.close()
+    return 0
+
+
 def run_single(args, torch, ctd, nat):
     seq, param, n0, n1, F0, F1, batch, desc = SINGLE[args.workload]
     N = n1 - n0
@@ -379,7 +520,11 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (mixed: 1, single-job workloads: 30)")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="mixed", choices=["mixed"] + sorted(SINGLE))
+    ap.add_argument("--workload", default="mixed", choices=["mixed", "serve"] + sorted(SINGLE))
+    ap.add_argument("--serve-jobs", type=int, default=10000, help="serve: jobs per offered load")
+    ap.add_argument("--loads", default="0.5,0.7,0.9", help="serve: offered loads (fractions of capacity)")
+    ap.add_argument("--capacity-jobs", type=int, default=2000, help="serve: jobs of the capacity measurement")
+    ap.add_argument("--serve-streams", type=int, default=4, help="serve: streams (resident prime slots)")
     ap.add_argument("--jobs", type=int, default=256, help="mixed: jobs drawn from the seeded generator")
     ap.add_argument("--tile-bits", type=int, default=0, help="mixed: log2 of the tile length (0 = engine default)")
     ap.add_argument("--no-autotune", action="store_true")
@@ -402,6 +547,8 @@ def main() -> int:
     import cutrialdive_amd as ctd
     from cutrialdive_amd import _native as nat
 
+    if args.workload == "serve":
+        return run_serve(args, torch, ctd, nat)
     if args.workload == "mixed":
         if args.steps is None:
             args.steps = 1

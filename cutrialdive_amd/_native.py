@@ -24,9 +24,12 @@ CTD_E_UNSUPPORTED = -4
 CTD_E_BUSY = -5
 
 CTD_RUN_GENERIC_ONLY = 1
+CTD_RUN_NO_FP64 = 2
 CTD_BATCH_HOST_BUFFERS = 1
 CTD_BATCH_NO_GRAPH = 2
 CTD_BATCH_NO_AUTOTUNE = 4
+CTD_BATCH_NO_FP64 = 8
+CTD_SERVE_FRESH_BUFFERS = 1
 
 
 class CtdJob(ctypes.Structure):
@@ -96,7 +99,8 @@ class CtdBatchStats(ctypes.Structure):
 
 
 class CtdServerOpts(ctypes.Structure):
-    _fields_ = [("memory_budget", u64), ("tile_length", u64), ("streams", u32), ("queue_limit", u32)]
+    _fields_ = [("memory_budget", u64), ("tile_length", u64), ("streams", u32), ("queue_limit", u32),
+                ("flags", u32), ("reserved", u32)]
 
 
 class CtdServerStats(ctypes.Structure):

@@ -29,6 +29,8 @@ HIP_FLAGS = [
     "-target-feature",
     "-Xclang",
     "-packed-fp32-ops",
+    # the FP64 residue forms rely on exact Dekker products: no implicit fma contraction
+    "-ffp-contract=off",
     f"-I{INCLUDE}",
     f"-I{CSRC}",
 ]

@@ -94,6 +94,7 @@ struct JobPlan {
     bool has2 = false;
     LaunchShape shape;
     double ops = 7.0;  // relative cost per update, for ordering tiles
+    bool fp64 = true;  // the FP64 forms are allowed
 };
 
 struct Tile {
@@ -128,6 +129,7 @@ JobArgs tile_args(const ctd_hip_engine *E, const JobPlan &P, uint64_t est_np, ui
     J.chunk = chunk;
     J.nchunks = (uint32_t)div_up(P.N, chunk);
     J.flags = (J.fam == FAM_SMARANDACHE && sm_narrow_ok(J.param, J.n1, pmax)) ? JOB_SM_NARROW : 0u;
+    if (pmax < FP_PMAX && P.fp64) J.flags |= JOB_FP64;
     return J;
 }
 
@@ -389,6 +391,8 @@ int run_batch(ctd_hip_engine *E, const ctd_job *jobs, uint32_t njobs, uint32_t *
     auto t_tune = clk::now();
     int rc = plan_jobs(E, jobs, njobs, dc.data(), df.data(), !(flags & CTD_BATCH_NO_AUTOTUNE), plans, &bs.tuned);
     if (rc != CTD_OK) return rc;
+    if (flags & CTD_BATCH_NO_FP64)
+        for (auto &P : plans) P.fp64 = false;
     bs.autotune_ms = std::chrono::duration<double, std::milli>(clk::now() - t_tune).count();
 
     // tiles, longest first, each to the least-loaded stream

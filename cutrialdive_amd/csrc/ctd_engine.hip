@@ -112,6 +112,12 @@ __global__ void zero_u64_kernel(unsigned long long *p, uint32_t n)
     if (i < n) p[i] = 0;
 }
 
+__global__ void zero_words_kernel(uint32_t *p, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
 // After a tile's sieve: keep its prime count and largest prime (or add the
 // count to a running total), and flag a count above the slot's capacity (the
 // sieve never writes past it).
@@ -512,6 +518,7 @@ int run_job(ctd_hip_engine *E, const ctd_job *job, const ctd_run_opts *opts, con
                 J.chunk = chunk;
                 J.nchunks = (uint32_t)nch;
                 J.flags = (J.fam == FAM_SMARANDACHE && sm_narrow_ok(J.param, J.n1, bhi - 1)) ? JOB_SM_NARROW : 0u;
+                if (bhi - 1 < FP_PMAX && !(opts && (opts->flags & CTD_RUN_NO_FP64))) J.flags |= JOB_FP64;
                 uint32_t block = (opts && opts->block_size > 0) ? (uint32_t)opts->block_size : 256u;
                 block = std::max<uint32_t>(64, std::min<uint32_t>(block, 256)) & ~63u;
                 uint64_t items = ((np + 63) & ~63ull) * nch;
@@ -911,6 +918,7 @@ int ctd_hip_residues(ctd_hip_engine *E, const ctd_job *job, const uint64_t *prim
         J.final_res = dr;
         const bool fast = !(flags & CTD_RUN_GENERIC_ONLY) && Fast<0>::fits(pmax);
         J.flags = (J.fam == FAM_SMARANDACHE && sm_narrow_ok(J.param, J.n1, pmax)) ? JOB_SM_NARROW : 0u;
+        if (pmax < FP_PMAX && !(flags & CTD_RUN_NO_FP64)) J.flags |= JOB_FP64;
         const uint64_t items = (nprimes + 63) & ~63ull;
         launch_tf_any(J, E->d_primes, nprimes, nullptr, fast, dim3((uint32_t)div_up(items, 256)), dim3(256),
                       E->stream);

@@ -24,7 +24,42 @@
 constexpr uint64_t SERVE_TILE_DEFAULT = 1ull << 26;
 constexpr uint64_t SERVE_BUDGET_DEFAULT = 8ull << 30;
 constexpr uint32_t SERVE_QUEUE_DEFAULT = 65536;
-constexpr uint32_t SERVE_DEPTH = 2; // tiles in flight per stream
+constexpr uint32_t SERVE_DEPTH = 4; // tiles in flight per stream
+
+// Device buffers of finished jobs are kept for later ones, by power-of-two
+// size class: hipFree can wait for the whole device, which would serialize
+// the streams at every job completion.
+struct DevicePool {
+    std::multimap<uint64_t, void *> free_bufs;
+    std::vector<void *> all;
+    static uint64_t size_class(uint64_t bytes)
+    {
+        uint64_t c = 4096;
+        while (c < bytes) c <<= 1;
+        return c;
+    }
+    void *get(uint64_t bytes)
+    {
+        const uint64_t c = size_class(bytes);
+        auto it = free_bufs.find(c);
+        if (it != free_bufs.end()) {
+            void *p = it->second;
+            free_bufs.erase(it);
+            return p;
+        }
+        void *p = nullptr;
+        HIPCHK(hipMalloc(&p, c));
+        all.push_back(p);
+        return p;
+    }
+    void put(void *p, uint64_t bytes) { free_bufs.emplace(size_class(bytes), p); }
+    void release()
+    {
+        for (void *p : all) (void)hipFree(p);
+        all.clear();
+        free_bufs.clear();
+    }
+};
 
 struct ServeJob {
     uint64_t ticket = 0;
@@ -62,6 +97,7 @@ struct ctd_hip_server_impl {
     std::vector<std::deque<InFlight>> inflight;
     hipStream_t copy_st = nullptr;
     std::vector<hipEvent_t> event_pool;
+    DevicePool pool;
 
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -126,14 +162,23 @@ bool admit(ctd_hip_server_impl *S, ServeJob *J)
         // the sieving base grows: no tile may be reading the old one
         for (auto &sl : S->slots) HIPCHK(hipStreamSynchronize(sl.st));
     }
-    HIPCHK(hipMalloc((void **)&J->d_counts, N * sizeof(uint32_t)));
-    HIPCHK(hipMalloc((void **)&J->d_factors, N * (uint64_t)q.max_factors * sizeof(uint64_t)));
-    HIPCHK(hipMalloc((void **)&J->d_acc, 2 * sizeof(unsigned long long)));
+    J->d_counts = (uint32_t *)S->pool.get(N * sizeof(uint32_t));
+    J->d_factors = (uint64_t *)S->pool.get(N * (uint64_t)q.max_factors * sizeof(uint64_t));
+    J->d_acc = (unsigned long long *)S->pool.get(2 * sizeof(unsigned long long));
     S->used_bytes += J->bytes;
     hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, S->copy_st, J->d_acc, 2u);
-    HIPCHK(hipMemcpyAsync(J->d_counts, J->h_counts, N * sizeof(uint32_t), hipMemcpyHostToDevice, S->copy_st));
-    HIPCHK(hipMemcpyAsync(J->d_factors, J->h_factors, N * (uint64_t)q.max_factors * sizeof(uint64_t),
-                          hipMemcpyHostToDevice, S->copy_st));
+    if (S->opts.flags & CTD_SERVE_FRESH_BUFFERS) {
+        // the job starts from an empty factors buffer: clear it on the device
+        const uint64_t wf = 2 * N * (uint64_t)q.max_factors;
+        hipLaunchKernelGGL(zero_words_kernel, dim3((uint32_t)std::min<uint64_t>(div_up(N, 256), 4096)), dim3(256), 0,
+                           S->copy_st, (uint32_t *)J->d_counts, N);
+        hipLaunchKernelGGL(zero_words_kernel, dim3((uint32_t)std::min<uint64_t>(div_up(wf, 1024), 4096)), dim3(256),
+                           0, S->copy_st, (uint32_t *)J->d_factors, wf);
+    } else {
+        HIPCHK(hipMemcpyAsync(J->d_counts, J->h_counts, N * sizeof(uint32_t), hipMemcpyHostToDevice, S->copy_st));
+        HIPCHK(hipMemcpyAsync(J->d_factors, J->h_factors, N * (uint64_t)q.max_factors * sizeof(uint64_t),
+                              hipMemcpyHostToDevice, S->copy_st));
+    }
     std::vector<JobPlan> plans;
     uint32_t tuned = 0;
     int rc = plan_jobs(E, &q, 1, &J->d_counts, &J->d_factors, false, plans, &tuned);
@@ -254,9 +299,9 @@ bool schedule_once(ctd_hip_server_impl *S)
             continue;
         }
         if (e != hipSuccess) throw HipError{e, "copy event"};
-        HIPCHK(hipFree(J->d_counts));
-        HIPCHK(hipFree(J->d_factors));
-        HIPCHK(hipFree(J->d_acc));
+        S->pool.put(J->d_counts, N * sizeof(uint32_t));
+        S->pool.put(J->d_factors, N * (uint64_t)J->job.max_factors * sizeof(uint64_t));
+        S->pool.put(J->d_acc, 2 * sizeof(unsigned long long));
         S->used_bytes -= J->bytes;
         S->event_pool.push_back(J->ready);
         S->event_pool.push_back(J->done);
@@ -302,7 +347,7 @@ void serve_loop(ctd_hip_server_impl *S)
                 S->cv_work.wait(lk, [&] { return S->stop || !S->pending.empty(); });
             } else {
                 // device work in flight: poll its events again shortly
-                S->cv_work.wait_for(lk, std::chrono::microseconds(50));
+                S->cv_work.wait_for(lk, std::chrono::microseconds(10));
             }
         }
     } catch (const HipError &h) {
@@ -328,4 +373,5 @@ void release_server(ctd_hip_server_impl *S)
         (void)hipStreamDestroy(S->copy_st);
     }
     for (auto e : S->event_pool) (void)hipEventDestroy(e);
+    S->pool.release();
 }

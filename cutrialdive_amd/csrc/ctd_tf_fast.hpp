@@ -119,6 +119,28 @@ __device__ __forceinline__ uint32_t add32(uint32_t a, uint32_t b, uint32_t q)
         }                                                                                          \
     }
 
+} // namespace ctd
+
+#include "ctd_tf_fp.hpp"
+
+namespace ctd {
+
+// The FP64 form (ctd_tf_fp.hpp) takes a wave when the host allows it
+// (JOB_FP64: every prime of the batch below FP_PMAX) and it is the cheaper
+// form for the wave: the 32-bit integer steps stay cheaper for doubling,
+// fixed-n and Fibonacci/Lucas when every prime is below 2^31; Smarandache
+// needs every prime at least FP_SM_PMIN and the narrow multiplier bound.
+template <uint32_t FAM>
+__device__ __forceinline__ bool use_fp(const JobArgs &J, uint64_t p)
+{
+    if (!(J.flags & JOB_FP64) || __ballot(p >= FP_PMAX) != 0) return false;
+    if constexpr (FAM == FAM_PEANO) return false;
+    else if constexpr (FAM == FAM_TRIBONACCI) return true;
+    else if constexpr (FAM == FAM_SMARANDACHE)
+        return (J.flags & JOB_SM_NARROW) && __ballot(p < FP_SM_PMIN) == 0;
+    else return !wave_w32(p);
+}
+
 template <uint32_t FAM>
 struct Fast {
     __host__ __device__ static bool fits(uint64_t pmax) { return pmax < FAST_PMAX; }
@@ -130,6 +152,10 @@ struct Fast {
         auto emit = [&](uint64_t v) {
             if (J.final_res) J.final_res[pi] = v;
         };
+        if (use_fp<FAM>(J, p)) {
+            emit(run_fp<FAM>(J, p, ns, ne, ns - J.n0, ne - ns - 1));
+            return;
+        }
         Mont M;
         M.init(p);
         const uint64_t steps = ne - ns - 1;

@@ -1,0 +1,308 @@
+// ctd_tf_fp.hpp -- residue propagation in exact FP64 arithmetic, p < 2^48.
+//
+// On gfx950 a 64-bit integer add or subtract is two VALU instructions (low
+// word, then high word with carry) while an FP64 add or fused multiply-add is
+// one, at the same full rate (the engine's probe: v_add_f64 37.8 T, v_fma_f64
+// 34.2 T lane-ops/s).  Below 2^53 FP64 arithmetic on integers is exact, so a
+// residue can live in a double:
+//
+//   * values are integers kept in the signed range |x| <= 0.75 p;
+//   * red(v) = v - p round(v / p), with round(v/p) = fma(v, 1/p, 1.5 2^52) - 1.5 2^52
+//     (exact integer result; 1/p is only an estimate, so round() may be off
+//     by one near a half-integer, which leaves |red(v)| <= 0.75 p);
+//   * mul(a, b) = hi - p round(hi/p) + lo with hi = ab rounded and
+//     lo = fma(a, b, -hi) its exact error (Dekker's product): the value is
+//     congruent to ab, and every operation is exact because each result fits
+//     in 53 bits (|ab| < 2^100, |ab/p| < 2^50, |hi - p round(hi/p)| < 2p).
+//
+// Every residue class has one representative with |x| < p, so after a
+// reduction the zero test is x == 0 and the doubling families test x == +-1.
+// Per index this costs (VALU instructions, compare included):
+//
+//   doubling x = 2x           5   (integer W64 form: 7)
+//   fixed n  x = x + c        5   (6)
+//   fibonacci / lucas         5   (6)
+//   tribonacci                6   (11.25)
+//   smarandache r m + n + 1   9   (12.25), for 2^35 <= p
+//
+// and the per-prime set-up (powers, doubling formulas, Smarandache jump) runs
+// on 6-instruction multiplies instead of ~40-instruction Montgomery products.
+#pragma once
+
+// included by ctd_tf_fast.hpp after record_hits and CTD_UNROLLED_LOOP
+
+namespace ctd {
+
+constexpr uint64_t FP_PMAX = 1ull << 48;   // every prime of the wave must be below
+constexpr uint64_t FP_SM_PMIN = 1ull << 35; // smarandache: r m + lo + n + 1 stays below 0.75 p
+
+struct Fmod {
+    double p, invp;
+    static constexpr double MAGIC = 6755399441055744.0; // 1.5 * 2^52: adding it rounds to an integer
+
+    __device__ __forceinline__ void init(uint64_t pi)
+    {
+        p = (double)pi;
+        invp = 1.0 / p;
+    }
+    // round(v / p) for |v / p| < 2^50, possibly off by one near a half-integer
+    __device__ __forceinline__ double quot(double v) const { return fma(v, invp, MAGIC) - MAGIC; }
+    __device__ __forceinline__ double red(double v) const { return fma(-quot(v), p, v); }
+    __device__ __forceinline__ double mul(double a, double b) const
+    {
+        const double hi = a * b;
+        const double lo = fma(a, b, -hi);
+        return fma(-quot(hi), p, hi) + lo;
+    }
+    __device__ __forceinline__ double add(double a, double b) const { return red(a + b); }
+    __device__ __forceinline__ double sub(double a, double b) const { return red(a - b); }
+    // the residue of an integer below 2^53
+    __device__ __forceinline__ double of(uint64_t x) const { return red((double)x); }
+    // 2^e (left-to-right square and double)
+    __device__ double pow2(uint64_t e) const
+    {
+        double x = 1.0;
+        if (e == 0) return x;
+        for (int b = 63 - __clzll(e); b >= 0; --b) {
+            x = mul(x, x);
+            if ((e >> b) & 1) x = red(x + x);
+        }
+        return x;
+    }
+    __device__ double pow(double x, uint64_t e) const
+    {
+        double r = 1.0;
+        while (e) {
+            if (e & 1) r = mul(r, x);
+            x = mul(x, x);
+            e >>= 1;
+        }
+        return r;
+    }
+    // canonical residue in [0, p) of |x| < 2^52
+    __device__ __forceinline__ uint64_t canon(double x, uint64_t pi) const
+    {
+        const int64_t t = (int64_t)red(x);
+        return t < 0 ? (uint64_t)(t + (int64_t)pi) : (uint64_t)t;
+    }
+};
+
+// (F(m), F(m+1)) by fast doubling
+__device__ inline void fib_pair_fp(const Fmod &F, uint64_t m, double &fa, double &fb)
+{
+    double a = 0.0, b = 1.0;
+    if (m) {
+        for (int bit = 63 - __clzll(m); bit >= 0; --bit) {
+            const double c = F.mul(a, F.sub(b + b, a));        // F(2k)
+            const double d = F.add(F.mul(a, a), F.mul(b, b));   // F(2k+1)
+            if ((m >> bit) & 1) {
+                a = d;
+                b = F.add(c, d);
+            } else {
+                a = c;
+                b = d;
+            }
+        }
+    }
+    fa = a;
+    fb = b;
+}
+
+// (T(n), T(n+1), T(n+2)), the doubling identities of fib_pair's tribonacci
+// counterpart trib_triple (ctd_families.hpp)
+__device__ inline void trib_triple_fp(const Fmod &F, uint64_t n, double &ta, double &tb, double &tc)
+{
+    double a = 0.0, b = 0.0, c = 1.0;
+    if (n) {
+        for (int bit = 63 - __clzll(n); bit >= 0; --bit) {
+            const double aa = F.mul(a, a), bb = F.mul(b, b), cc = F.mul(c, c);
+            const double ab2 = F.mul(a, b + b), ac2 = F.mul(a, c + c), bc2 = F.mul(b, c + c);
+            const double t0 = F.red(ac2 + bb - ab2 - aa); // T(2k)
+            const double t1 = F.red(aa + bc2 - bb);       // T(2k+1)
+            const double t2 = F.red(ab2 + cc + bb);       // T(2k+2)
+            if ((n >> bit) & 1) {
+                a = t1;
+                b = t2;
+                c = F.red(t0 + t1 + t2);
+            } else {
+                a = t0;
+                b = t1;
+                c = t2;
+            }
+        }
+    }
+    ta = a;
+    tb = b;
+    tc = c;
+}
+
+// Sm_b(n) mod p by affine block jumps (the same decomposition as sm_value in
+// ctd_families.hpp): appending t consecutive d-digit numbers m0..m0+t-1 maps S
+// to B^t S + m0 U_t + V_t.
+__device__ inline double sm_value_fp(const Fmod &F, uint64_t b, uint64_t n)
+{
+    if (n == 0) return 0.0;
+    const double bd = (double)b;
+    double S = 0.0;
+    uint64_t m0 = 1;
+    double m0F = 1.0, BF = F.of(b);
+    for (;;) {
+        const uint64_t m1 = sat_mul(m0, b);
+        const uint64_t last = n < m1 - 1 ? n : m1 - 1;
+        const uint64_t t = last - m0 + 1;
+        double P = 1.0, U = 0.0, V = 0.0, tF = 0.0;
+        for (int bit = 63 - __clzll(t); bit >= 0; --bit) {
+            V = F.red(F.mul(P, V) + F.mul(tF, U) + V);
+            U = F.mul(U, P + 1.0);
+            P = F.mul(P, P);
+            tF = F.red(tF + tF);
+            if ((t >> bit) & 1) {
+                V = F.add(F.mul(BF, V), tF);
+                U = F.add(F.mul(U, BF), 1.0);
+                P = F.mul(P, BF);
+                tF = F.add(tF, 1.0);
+            }
+        }
+        S = F.red(F.mul(P, S) + F.mul(m0F, U) + V);
+        if (last == n || m1 == UINT64_MAX) break;
+        m0 = m1;
+        m0F = F.mul(m0F, bd);
+        BF = F.mul(BF, bd);
+    }
+    return S;
+}
+
+// The FP64 form of Fast<FAM>::run for a wave whose primes are all below
+// FP_PMAX (and, for smarandache, at least FP_SM_PMIN).  Returns the plain
+// residue S(ne-1) mod p for the test hook.
+template <uint32_t FAM>
+__device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t ne, uint64_t idx, uint64_t steps)
+{
+    Fmod F;
+    F.init(p);
+    if constexpr (FAM == FAM_MERSENNE || FAM == FAM_PROTH || FAM == FAM_RIESEL) {
+        // x = c 2^n, hit when x == target (the representative of +-1 is +-1)
+        double x = F.pow2(ns);
+        if constexpr (FAM != FAM_MERSENNE) x = F.mul(F.of(J.param < p ? J.param : J.param % p), x);
+        x = F.red(x);
+        const double tgt = (FAM == FAM_PROTH) ? -1.0 : 1.0;
+        record_hits(J, idx, p, x == tgt);
+        CTD_UNROLLED_LOOP(
+            {
+                x = F.red(x + x); h0 = x == tgt;
+                x = F.red(x + x); h1 = x == tgt;
+                x = F.red(x + x); h2 = x == tgt;
+                x = F.red(x + x); h3 = x == tgt;
+            },
+            x = F.red(x + x), x == tgt)
+        return F.canon(FAM == FAM_PROTH ? x + 1.0 : x - 1.0, p);
+    } else if constexpr (FAM == FAM_PROTH_FIXED_N || FAM == FAM_RIESEL_FIXED_N) {
+        // x = S(k) = k 2^n +- 1; step x += 2^n
+        const double c = F.pow2(J.param);
+        const uint64_t k = ns < (1ull << 52) ? ns : ns % p;
+        double x = F.red(F.mul(F.of(k), c) + (FAM == FAM_PROTH_FIXED_N ? 1.0 : -1.0));
+        record_hits(J, idx, p, x == 0.0);
+        CTD_UNROLLED_LOOP(
+            {
+                x = F.red(x + c); h0 = x == 0.0;
+                x = F.red(x + c); h1 = x == 0.0;
+                x = F.red(x + c); h2 = x == 0.0;
+                x = F.red(x + c); h3 = x == 0.0;
+            },
+            x = F.red(x + c), x == 0.0)
+        return F.canon(x, p);
+    } else if constexpr (FAM == FAM_FIBONACCI || FAM == FAM_LUCAS) {
+        double a = 0.0, b = 0.0; // (S(n-1), S(n)); n0 = 0 keeps {0, 0} (fibonacci.hpp:68-71)
+        if (J.n0 != 0) {
+            double fa, fb;
+            fib_pair_fp(F, ns - 1, fa, fb);
+            if constexpr (FAM == FAM_FIBONACCI) {
+                a = fa;
+                b = fb;
+            } else { // L(n-1) = 2F(n) - F(n-1), L(n) = F(n) + 2F(n-1)
+                a = F.red(fb + fb - fa);
+                b = F.red(fb + fa + fa);
+            }
+        }
+        record_hits(J, idx, p, b == 0.0);
+        CTD_UNROLLED_LOOP(
+            {
+                a = F.red(a + b); h0 = a == 0.0;
+                b = F.red(a + b); h1 = b == 0.0;
+                a = F.red(a + b); h2 = a == 0.0;
+                b = F.red(a + b); h3 = b == 0.0;
+            },
+            { const double t = F.red(a + b); a = b; b = t; }, b == 0.0)
+        return F.canon(b, p);
+    } else if constexpr (FAM == FAM_TRIBONACCI) {
+        double a, b, c;
+        trib_triple_fp(F, ns, a, b, c);
+        record_hits(J, idx, p, a == 0.0);
+        CTD_UNROLLED_LOOP(
+            {
+                a = F.red(a + b + c); h0 = b == 0.0;
+                b = F.red(b + c + a); h1 = c == 0.0;
+                c = F.red(c + a + b); h2 = a == 0.0;
+                const double t = F.red(a + b + c);
+                a = b; b = c; c = t; h3 = a == 0.0;
+            },
+            { const double t = F.red(a + b + c); a = b; b = c; c = t; }, a == 0.0)
+        return F.canon(a, p);
+    } else if constexpr (FAM == FAM_SMARANDACHE) {
+        // r = S(n); step r = r m + (n+1) with m = b^digits(n+1) < 2^21 (host
+        // flag JOB_SM_NARROW) and n+1 < 2^32: hi - p round(hi/p) is within
+        // 0.5 p, lo within 2^16, so |r| < 0.5 p + 2^33 <= 0.75 p for p >= 2^35
+        // and S(n) = 0 mod p  <=>  r == 0.
+        const uint64_t bb = J.param;
+        double r = sm_value_fp(F, bb, ns);
+        record_hits(J, idx, p, r == 0.0);
+        uint32_t d;
+        uint64_t nxt;
+        digits_pow(bb, ns + 1, d, nxt);
+        double m = 1.0;
+        for (uint32_t i = 0; i < d; ++i) m *= (double)bb;
+        double c = (double)(ns + 1); // the number appended by the next step (wave-uniform)
+        auto step = [&](void) {
+            const double hi = r * m;
+            const double lo = fma(r, m, -hi);
+            r = fma(-F.quot(hi), F.p, hi) + (lo + c);
+            c += 1.0;
+        };
+        uint64_t n = ns;
+        while (n + 1 < ne) {
+            const uint64_t seg_last = nxt - 2 < ne - 1 ? nxt - 2 : ne - 2; // last n of a step with this m
+            const uint64_t cnt = seg_last - n + 1;
+            uint64_t kk = 0;
+            for (; kk + UNROLL <= cnt; kk += UNROLL) {
+                step(); const uint64_t m0 = __ballot(r == 0.0);
+                step(); const uint64_t m1 = __ballot(r == 0.0);
+                step(); const uint64_t m2 = __ballot(r == 0.0);
+                step(); const uint64_t m3 = __ballot(r == 0.0);
+                if (m0 | m1 | m2 | m3) {
+                    const uint32_t lane = __lane_id();
+                    if (m0) record_hits_mask(J, idx + 1, p, (m0 >> lane) & 1, m0);
+                    if (m1) record_hits_mask(J, idx + 2, p, (m1 >> lane) & 1, m1);
+                    if (m2) record_hits_mask(J, idx + 3, p, (m2 >> lane) & 1, m2);
+                    if (m3) record_hits_mask(J, idx + 4, p, (m3 >> lane) & 1, m3);
+                }
+                idx += UNROLL;
+            }
+            for (; kk < cnt; ++kk) {
+                step();
+                ++idx;
+                const uint64_t mk = __ballot(r == 0.0);
+                if (mk) record_hits_mask(J, idx, p, (mk >> __lane_id()) & 1, mk);
+            }
+            n = seg_last + 1;
+            if (n + 1 < ne) { // the next appended number has one more digit
+                m *= (double)bb;
+                nxt = sat_mul(nxt, bb);
+            }
+        }
+        return F.canon(r, p);
+    } else {
+        return 0;
+    }
+}
+
+} // namespace ctd

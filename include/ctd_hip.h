@@ -82,6 +82,7 @@ typedef struct ctd_run_opts {
 } ctd_run_opts;
 
 #define CTD_RUN_GENERIC_ONLY 1u /* force the generic 64-bit Montgomery kernels */
+#define CTD_RUN_NO_FP64 2u      /* integer forms only (no FP64 residues below 2^48) */
 
 typedef struct ctd_hip_engine ctd_hip_engine;
 
@@ -152,6 +153,7 @@ int ctd_hip_last_stats(const ctd_hip_engine *eng, ctd_run_stats *out);
 #define CTD_BATCH_HOST_BUFFERS 1u /* counts/factors are host arrays (copied in and back) */
 #define CTD_BATCH_NO_GRAPH 2u     /* issue the tiles eagerly on the streams (no graph) */
 #define CTD_BATCH_NO_AUTOTUNE 4u  /* fixed shape: 256-lane groups, one item per lane, no extra chunking */
+#define CTD_BATCH_NO_FP64 8u      /* integer residue forms only */
 
 typedef struct ctd_batch_opts {
     uint64_t tile_length;  /* integers of a job's prime window per tile (0 = 2^28) */
@@ -200,7 +202,11 @@ typedef struct ctd_server_opts {
     uint64_t tile_length;   /* integers of a prime window per tile (0 = 2^26) */
     uint32_t streams;       /* streams = resident prime slots (0 = 4, at most 16) */
     uint32_t queue_limit;   /* jobs waiting for admission before submit returns CTD_E_BUSY (0 = 65536) */
+    uint32_t flags;         /* CTD_SERVE_* */
+    uint32_t reserved;
 } ctd_server_opts;
+#define CTD_SERVE_FRESH_BUFFERS 1u /* jobs start from empty factors buffers: the host
+                                      buffers are only written, never read */
 
 typedef struct ctd_server_stats {
     uint64_t submitted, admitted, completed, failed;

@@ -114,6 +114,26 @@ def test_family_w64_window(engine, seq, param):
 
 
 @pytest.mark.parametrize("seq,param", FAMILIES)
+def test_family_w64_window_integer_forms(engine, seq, param):
+    # the same window without the FP64 forms
+    n0, n1 = 100, 400
+    f0, f1 = 1 << 36, (1 << 36) + (1 << 16)
+    g, _, _ = gpu_lists(engine, seq, param, n0, n1, f0, f1, flags=nat.CTD_RUN_NO_FP64)
+    r = o.trial_factor(seq, param, n0, n1, f0, f1)
+    assert g == r
+
+
+@pytest.mark.parametrize("seq,param", FAMILIES)
+def test_family_w42_window(engine, seq, param):
+    # primes near 2^42 with large indices: FP64 set-up and loops
+    n0, n1 = 40000, 40600
+    f0, f1 = (1 << 42) + (1 << 30), (1 << 42) + (1 << 30) + (1 << 17)
+    g, _, _ = gpu_lists(engine, seq, param, n0, n1, f0, f1)
+    r = o.trial_factor(seq, param, n0, n1, f0, f1)
+    assert g == r
+
+
+@pytest.mark.parametrize("seq,param", FAMILIES)
 def test_family_chunked(engine, seq, param):
     # index chunks start mid-range: jump-ahead must equal propagation
     n0, n1 = 7, 3000

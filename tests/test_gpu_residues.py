@@ -3,8 +3,8 @@ index range, S(n1-1) mod p, must equal the oracle's for every prime.
 
 Zero residues (the factors) are rare for large primes, so factor-list
 comparisons alone leave most of the arithmetic unobserved; the final residue
-exposes every step.  Each case runs the fast kernel and the generic
-(64-bit Montgomery) kernel.
+exposes every step.  Each case runs the default kernels (FP64 forms below
+2^48), the integer forms only, and the generic (64-bit Montgomery) kernel.
 """
 import random
 
@@ -38,9 +38,10 @@ FAMILIES = [
 ]
 
 # prime-size classes: 32-bit residues, 2^31..2^32, the narrow Smarandache
-# range, the config-3/4 range and near 2^63 / 2^64 (generic only above 2^63)
-WINDOWS = [(3, 1 << 20), (1 << 31, 1 << 32), (1 << 32, 1 << 36), (1 << 38, 1 << 43), (1 << 62, 1 << 63),
-           ((1 << 63) + 1, (1 << 64) - 1)]
+# range (FP64 Smarandache from 2^35), the config-3/4 range, both sides of the
+# FP64 limit 2^48, and near 2^63 / 2^64 (generic only above 2^63)
+WINDOWS = [(3, 1 << 20), (1 << 31, 1 << 32), (1 << 32, 1 << 36), (1 << 38, 1 << 43), (1 << 47, 1 << 48),
+           (1 << 48, 1 << 49), (1 << 62, 1 << 63), ((1 << 63) + 1, (1 << 64) - 1)]
 
 
 def random_primes(lo, hi, k, seed):
@@ -58,7 +59,7 @@ def test_final_residues(engine, seq, param, win):
     primes = random_primes(lo, hi, 256, seed=seq * 97 + win)
     n0, n1 = (5, 2100) if seq != o.PEANO else (10 ** 12, 10 ** 12 + 500)
     want = o.residues_last(seq, param, n0, n1 - n0, primes)
-    for flags in (0, nat.CTD_RUN_GENERIC_ONLY):
+    for flags in (0, nat.CTD_RUN_NO_FP64, nat.CTD_RUN_GENERIC_ONLY):
         got = engine.residues(seq, param, n0, n1, primes, flags)
         bad = np.nonzero(got != want)[0]
         assert len(bad) == 0, (flags, [(primes[i], int(got[i]), int(want[i])) for i in bad[:5]])

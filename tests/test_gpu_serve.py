@@ -124,3 +124,17 @@ def test_memory_budget_and_queue_limit(engine):
             srv.wait(t)
         st = srv.stats()
         assert st.active_max == 1 and st.pending_max == 2
+
+
+def test_fresh_buffers_ignore_host_contents(engine):
+    # pooled device buffers are reused: a fresh job must not see an earlier job's factors
+    first = workloads.JobSpec("x", 0, 0, 2, 400, 3, 1 << 22, 16)
+    second = workloads.JobSpec("x", 1, 3, 2, 400, 3, 1 << 22, 16)
+    with ctd.Server(engine, fresh_buffers=True) as srv:
+        srv.wait(srv.submit(job_of(first)))
+        junk_c = np.full(398, 7, np.uint32)
+        junk_f = np.full(398 * 16, 5, np.uint64)
+        c, f, _ = srv.wait(srv.submit(job_of(second), junk_c, junk_f))
+    rc, rf = oracle.trial_factor_raw(second.seq_id, second.param, second.n0, second.n1, second.f0, second.f1, 16)
+    assert np.array_equal(c, rc) and np.array_equal(np.sort(f.reshape(398, 16), axis=1),
+                                                    np.sort(rf.reshape(398, 16), axis=1))
