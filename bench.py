@@ -72,16 +72,17 @@ def ops_per_update(workload: str, seq: int, pmax: int):
     return float(fam["w32"] if pmax < (1 << 31) else fam["w64"]), "loop ISA count"
 
 
-def mixed_ops(jobs):
+def mixed_ops(jobs, full_batch=True):
     """Update-weighted VALU lane-ops per update of the mixed batch."""
     t = ops_table()
     m = t["measured"].get("mixed")
-    if m:
+    if m and full_batch:
         return float(m["valu_lane_ops_per_update"]), "rocprofv3 " + m["profile"]
     num = den = 0.0
     for j in jobs:
         u = (j.f1 - j.f0) / math.log(j.f1) * (j.n1 - j.n0)
-        num += u * float(t["loop"][FAMILY_NAMES[j.seq_id]]["w64"])
+        form = "fp64" if j.f1 <= (1 << 48) else "w64"
+        num += u * float(t["loop"][FAMILY_NAMES[j.seq_id]][form])
         den += u
     return num / den, "loop ISA count, update-weighted"
 
@@ -195,7 +196,7 @@ def run_mixed(args, torch, ctd, nat):
                         "factors_identical_to_batched": bool(match)}
         print(f"# plain form {pdt:.1f} s, identical factors: {match}", file=sys.stderr, flush=True)
 
-    c_upd, c_src = mixed_ops(jobs)
+    c_upd, c_src = mixed_ops(jobs, full_batch=args.family is None and args.jobs == 256)
     peak = eng.valu_probe(0, 4096) / 1e12
     per_launch_s = bms / 1e3 / max(graphs, 1)
     achieved = upd / max(graphs, 1) * c_upd / per_launch_s / 1e12

@@ -119,6 +119,31 @@ __device__ __forceinline__ uint32_t add32(uint32_t a, uint32_t b, uint32_t q)
         }                                                                                          \
     }
 
+// The same with the step computing the wave masks m0..m3 itself (a hit that
+// is the OR of two compares stays a pair of scalar masks).
+#define CTD_UNROLLED_LOOP_MASKS(STEP4, STEP1, MASK1)                                             \
+    {                                                                                              \
+        uint64_t k = 0;                                                                            \
+        for (; k + UNROLL <= steps; k += UNROLL) {                                                 \
+            uint64_t m0, m1, m2, m3;                                                               \
+            STEP4;                                                                                 \
+            if (m0 | m1 | m2 | m3) {                                                               \
+                const uint32_t lane = __lane_id();                                                 \
+                if (m0) record_hits_mask(J, idx + 1, p, (m0 >> lane) & 1, m0);                     \
+                if (m1) record_hits_mask(J, idx + 2, p, (m1 >> lane) & 1, m1);                     \
+                if (m2) record_hits_mask(J, idx + 3, p, (m2 >> lane) & 1, m2);                     \
+                if (m3) record_hits_mask(J, idx + 4, p, (m3 >> lane) & 1, m3);                     \
+            }                                                                                      \
+            idx += UNROLL;                                                                         \
+        }                                                                                          \
+        for (; k < steps; ++k) {                                                                   \
+            STEP1;                                                                                 \
+            ++idx;                                                                                 \
+            const uint64_t mk = MASK1;                                                             \
+            if (mk) record_hits_mask(J, idx, p, (mk >> __lane_id()) & 1, mk);                      \
+        }                                                                                          \
+    }
+
 } // namespace ctd
 
 #include "ctd_tf_fp.hpp"

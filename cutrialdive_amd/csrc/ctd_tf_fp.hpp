@@ -19,8 +19,8 @@
 // reduction the zero test is x == 0 and the doubling families test x == +-1.
 // Per index this costs (VALU instructions, compare included):
 //
-//   doubling x = 2x           5   (integer W64 form: 7)
-//   fixed n  x = x + c        5   (6)
+//   doubling x = 2x           4   (integer W64 form: 7); every other index
+//   fixed n  x = x + c        4   (6)   skips the reduction and compares twice
 //   fibonacci / lucas         5   (6)
 //   tribonacci                6   (11.25)
 //   smarandache r m + n + 1   9   (12.25), for 2^35 <= p
@@ -186,15 +186,18 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
         if constexpr (FAM != FAM_MERSENNE) x = F.mul(F.of(J.param < p ? J.param : J.param % p), x);
         x = F.red(x);
         const double tgt = (FAM == FAM_PROTH) ? -1.0 : 1.0;
+        // every other index skips the reduction: y = 2x lies in (-p, p), where
+        // the target class has the two representatives tgt and tgt -+ p
+        const double tgt2 = (FAM == FAM_PROTH) ? F.p - 1.0 : 1.0 - F.p;
         record_hits(J, idx, p, x == tgt);
-        CTD_UNROLLED_LOOP(
+        CTD_UNROLLED_LOOP_MASKS(
             {
-                x = F.red(x + x); h0 = x == tgt;
-                x = F.red(x + x); h1 = x == tgt;
-                x = F.red(x + x); h2 = x == tgt;
-                x = F.red(x + x); h3 = x == tgt;
+                double y = x + x; m0 = __ballot(y == tgt) | __ballot(y == tgt2);
+                x = F.red(y + y); m1 = __ballot(x == tgt);
+                y = x + x; m2 = __ballot(y == tgt) | __ballot(y == tgt2);
+                x = F.red(y + y); m3 = __ballot(x == tgt);
             },
-            x = F.red(x + x), x == tgt)
+            x = F.red(x + x), __ballot(x == tgt))
         return F.canon(FAM == FAM_PROTH ? x + 1.0 : x - 1.0, p);
     } else if constexpr (FAM == FAM_PROTH_FIXED_N || FAM == FAM_RIESEL_FIXED_N) {
         // x = S(k) = k 2^n +- 1; step x += 2^n
@@ -202,14 +205,16 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
         const uint64_t k = ns < (1ull << 52) ? ns : ns % p;
         double x = F.red(F.mul(F.of(k), c) + (FAM == FAM_PROTH_FIXED_N ? 1.0 : -1.0));
         record_hits(J, idx, p, x == 0.0);
-        CTD_UNROLLED_LOOP(
+        // every other index skips the reduction: y = x + c lies in (-p, p),
+        // where the zero class is {0, -p, p}
+        CTD_UNROLLED_LOOP_MASKS(
             {
-                x = F.red(x + c); h0 = x == 0.0;
-                x = F.red(x + c); h1 = x == 0.0;
-                x = F.red(x + c); h2 = x == 0.0;
-                x = F.red(x + c); h3 = x == 0.0;
+                double y = x + c; m0 = __ballot(y == 0.0) | __ballot(fabs(y) == F.p);
+                x = F.red(y + c); m1 = __ballot(x == 0.0);
+                y = x + c; m2 = __ballot(y == 0.0) | __ballot(fabs(y) == F.p);
+                x = F.red(y + c); m3 = __ballot(x == 0.0);
             },
-            x = F.red(x + c), x == 0.0)
+            x = F.red(x + c), __ballot(x == 0.0))
         return F.canon(x, p);
     } else if constexpr (FAM == FAM_FIBONACCI || FAM == FAM_LUCAS) {
         double a = 0.0, b = 0.0; // (S(n-1), S(n)); n0 = 0 keeps {0, 0} (fibonacci.hpp:68-71)
