@@ -124,7 +124,7 @@ JobArgs tile_args(const ctd_hip_engine *E, const JobPlan &P, uint64_t est_np, ui
     const uint64_t target = (uint64_t)E->cu * 4096;
     if (est_np < target && P.N > 2048) nch = std::min<uint64_t>(div_up(target, std::max<uint64_t>(est_np, 1)),
                                                                  div_up(P.N, 1024));
-    nch = std::min<uint64_t>(nch * P.shape.split, P.N);
+    nch = std::min<uint64_t>(std::max<uint64_t>(nch * P.shape.split, div_up(P.N, CHUNK_MAX)), P.N);
     const uint64_t chunk = div_up(P.N, nch);
     J.chunk = chunk;
     J.nchunks = (uint32_t)div_up(P.N, chunk);

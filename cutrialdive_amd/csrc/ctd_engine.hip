@@ -242,6 +242,9 @@ uint64_t isqrt_u64(uint64_t n)
 
 uint64_t div_up(uint64_t a, uint64_t b) { return a / b + (a % b != 0); }
 
+// Indices per work item at most: the index loops keep 32-bit counters.
+constexpr uint64_t CHUNK_MAX = 1ull << 30;
+
 // Sieves [start, end) into an ordered list (u32 or u64) with the current
 // sieving numbers sp/mu.  Returns the count; allocates *out32 or *out64.
 uint64_t sieve_ordered(ctd_hip_engine *E, const uint32_t *sp, const uint64_t *mu, uint32_t nsp, uint32_t nsmall,
@@ -515,8 +518,12 @@ int run_job(ctd_hip_engine *E, const ctd_job *job, const ctd_run_opts *opts, con
                         nch = div_up(N, chunk);
                     }
                 }
+                if (chunk > CHUNK_MAX) { // the index loops count in 32 bits
+                    nch = div_up(N, CHUNK_MAX);
+                    chunk = div_up(N, nch);
+                }
                 J.chunk = chunk;
-                J.nchunks = (uint32_t)nch;
+                J.nchunks = (uint32_t)div_up(N, chunk);
                 J.flags = (J.fam == FAM_SMARANDACHE && sm_narrow_ok(J.param, J.n1, bhi - 1)) ? JOB_SM_NARROW : 0u;
                 if (bhi - 1 < FP_PMAX && !(opts && (opts->flags & CTD_RUN_NO_FP64))) J.flags |= JOB_FP64;
                 uint32_t block = (opts && opts->block_size > 0) ? (uint32_t)opts->block_size : 256u;
@@ -911,8 +918,8 @@ int ctd_hip_residues(ctd_hip_engine *E, const ctd_job *job, const uint64_t *prim
         J.param = job->param;
         J.n0 = job->n0;
         J.n1 = job->n1;
-        J.chunk = N;
-        J.nchunks = 1;
+        J.chunk = std::min<uint64_t>(N, CHUNK_MAX);
+        J.nchunks = (uint32_t)div_up(N, J.chunk);
         J.counts = dc;
         J.factors = df;
         J.final_res = dr;

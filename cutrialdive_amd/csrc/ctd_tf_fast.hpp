@@ -93,26 +93,40 @@ __device__ __forceinline__ uint32_t add32(uint32_t a, uint32_t b, uint32_t q)
     return min(t, t - q);
 }
 
-// Runs `steps` steps of STEP(k) (which must set the bools h0..h3 for the 4
-// unrolled indices) with one wave-wide branch per group.  idx is the index of
-// the state before the group.
+// idx + k as an opaque scalar: the hit path computes its addresses from it,
+// so the compiler keeps no per-group pointer induction variables (scalar
+// work that would compete with the VALU stream).
+__device__ __forceinline__ uint64_t hit_base(uint64_t idx, uint32_t k)
+{
+    uint64_t at = idx + k;
+    asm volatile("" : "+s"(at));
+    return at;
+}
+
+// Runs `steps` steps with one wave-wide branch per UNROLL indices.  STEP4
+// sets the bools h0..h3 of the 4 unrolled indices; STEP1 does one step whose
+// hit is HIT1.  idx is the index (relative to n0) of the state before the
+// loop; the counter is 32-bit (work items hold fewer than 2^31 indices) so
+// the loop control stays three scalar instructions.
 #define CTD_UNROLLED_LOOP(STEP4, STEP1, HIT1)                                                    \
     {                                                                                              \
-        uint64_t k = 0;                                                                            \
-        for (; k + UNROLL <= steps; k += UNROLL) {                                                 \
+        const uint32_t steps32 = (uint32_t)steps;                                                 \
+        uint32_t k = 0;                                                                            \
+        for (; k + UNROLL <= steps32; k += UNROLL) {                                               \
             bool h0, h1, h2, h3;                                                                   \
             STEP4;                                                                                 \
             const uint64_t m0 = __ballot(h0), m1 = __ballot(h1), m2 = __ballot(h2),               \
                            m3 = __ballot(h3);                                                      \
             if (m0 | m1 | m2 | m3) {                                                               \
-                if (m0) record_hits_mask(J, idx + 1, p, h0, m0);                                   \
-                if (m1) record_hits_mask(J, idx + 2, p, h1, m1);                                   \
-                if (m2) record_hits_mask(J, idx + 3, p, h2, m2);                                   \
-                if (m3) record_hits_mask(J, idx + 4, p, h3, m3);                                   \
+                const uint64_t at = hit_base(idx, k);                                              \
+                if (m0) record_hits_mask(J, at + 1, p, h0, m0);                                    \
+                if (m1) record_hits_mask(J, at + 2, p, h1, m1);                                    \
+                if (m2) record_hits_mask(J, at + 3, p, h2, m2);                                    \
+                if (m3) record_hits_mask(J, at + 4, p, h3, m3);                                    \
             }                                                                                      \
-            idx += UNROLL;                                                                         \
         }                                                                                          \
-        for (; k < steps; ++k) {                                                                   \
+        idx += k;                                                                                  \
+        for (; k < steps32; ++k) {                                                                 \
             STEP1;                                                                                 \
             ++idx;                                                                                 \
             record_hits(J, idx, p, HIT1);                                                          \
@@ -123,20 +137,22 @@ __device__ __forceinline__ uint32_t add32(uint32_t a, uint32_t b, uint32_t q)
 // is the OR of two compares stays a pair of scalar masks).
 #define CTD_UNROLLED_LOOP_MASKS(STEP4, STEP1, MASK1)                                             \
     {                                                                                              \
-        uint64_t k = 0;                                                                            \
-        for (; k + UNROLL <= steps; k += UNROLL) {                                                 \
+        const uint32_t steps32 = (uint32_t)steps;                                                 \
+        uint32_t k = 0;                                                                            \
+        for (; k + UNROLL <= steps32; k += UNROLL) {                                               \
             uint64_t m0, m1, m2, m3;                                                               \
             STEP4;                                                                                 \
             if (m0 | m1 | m2 | m3) {                                                               \
                 const uint32_t lane = __lane_id();                                                 \
-                if (m0) record_hits_mask(J, idx + 1, p, (m0 >> lane) & 1, m0);                     \
-                if (m1) record_hits_mask(J, idx + 2, p, (m1 >> lane) & 1, m1);                     \
-                if (m2) record_hits_mask(J, idx + 3, p, (m2 >> lane) & 1, m2);                     \
-                if (m3) record_hits_mask(J, idx + 4, p, (m3 >> lane) & 1, m3);                     \
+                const uint64_t at = hit_base(idx, k);                                              \
+                if (m0) record_hits_mask(J, at + 1, p, (m0 >> lane) & 1, m0);                      \
+                if (m1) record_hits_mask(J, at + 2, p, (m1 >> lane) & 1, m1);                      \
+                if (m2) record_hits_mask(J, at + 3, p, (m2 >> lane) & 1, m2);                      \
+                if (m3) record_hits_mask(J, at + 4, p, (m3 >> lane) & 1, m3);                      \
             }                                                                                      \
-            idx += UNROLL;                                                                         \
         }                                                                                          \
-        for (; k < steps; ++k) {                                                                   \
+        idx += k;                                                                                  \
+        for (; k < steps32; ++k) {                                                                 \
             STEP1;                                                                                 \
             ++idx;                                                                                 \
             const uint64_t mk = MASK1;                                                             \
@@ -383,8 +399,8 @@ struct Fast {
         while (n + 1 < ne) {
             // indices n+1 .. seg_end use the current m (digits change when n+2 == nxt)
             uint64_t seg_last = nxt - 2 < ne - 1 ? nxt - 2 : ne - 2; // last n of a step with this m
-            uint64_t cnt = seg_last - n + 1;
-            uint64_t kk = 0;
+            const uint32_t cnt = (uint32_t)(seg_last - n + 1); // below 2^32: n1 <= 2^32
+            uint32_t kk = 0;
             for (; kk + UNROLL <= cnt; kk += UNROLL) {
                 step(); c++; const uint64_t m0 = hits();
                 step(); c++; const uint64_t m1 = hits();
@@ -392,13 +408,14 @@ struct Fast {
                 step(); c++; const uint64_t m3 = hits();
                 if (m0 | m1 | m2 | m3) {
                     const uint32_t lane = __lane_id();
-                    if (m0) record_hits_mask(J, idx + 1, p, (m0 >> lane) & 1, m0);
-                    if (m1) record_hits_mask(J, idx + 2, p, (m1 >> lane) & 1, m1);
-                    if (m2) record_hits_mask(J, idx + 3, p, (m2 >> lane) & 1, m2);
-                    if (m3) record_hits_mask(J, idx + 4, p, (m3 >> lane) & 1, m3);
+                    const uint64_t at = hit_base(idx, kk);
+                    if (m0) record_hits_mask(J, at + 1, p, (m0 >> lane) & 1, m0);
+                    if (m1) record_hits_mask(J, at + 2, p, (m1 >> lane) & 1, m1);
+                    if (m2) record_hits_mask(J, at + 3, p, (m2 >> lane) & 1, m2);
+                    if (m3) record_hits_mask(J, at + 4, p, (m3 >> lane) & 1, m3);
                 }
-                idx += UNROLL;
             }
+            idx += kk;
             for (; kk < cnt; ++kk) {
                 step();
                 c++;

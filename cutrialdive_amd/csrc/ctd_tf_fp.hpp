@@ -276,8 +276,8 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
         uint64_t n = ns;
         while (n + 1 < ne) {
             const uint64_t seg_last = nxt - 2 < ne - 1 ? nxt - 2 : ne - 2; // last n of a step with this m
-            const uint64_t cnt = seg_last - n + 1;
-            uint64_t kk = 0;
+            const uint32_t cnt = (uint32_t)(seg_last - n + 1); // below 2^32: n1 <= 2^32
+            uint32_t kk = 0;
             for (; kk + UNROLL <= cnt; kk += UNROLL) {
                 step(); const uint64_t m0 = __ballot(r == 0.0);
                 step(); const uint64_t m1 = __ballot(r == 0.0);
@@ -285,13 +285,14 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
                 step(); const uint64_t m3 = __ballot(r == 0.0);
                 if (m0 | m1 | m2 | m3) {
                     const uint32_t lane = __lane_id();
-                    if (m0) record_hits_mask(J, idx + 1, p, (m0 >> lane) & 1, m0);
-                    if (m1) record_hits_mask(J, idx + 2, p, (m1 >> lane) & 1, m1);
-                    if (m2) record_hits_mask(J, idx + 3, p, (m2 >> lane) & 1, m2);
-                    if (m3) record_hits_mask(J, idx + 4, p, (m3 >> lane) & 1, m3);
+                    const uint64_t at = hit_base(idx, kk);
+                    if (m0) record_hits_mask(J, at + 1, p, (m0 >> lane) & 1, m0);
+                    if (m1) record_hits_mask(J, at + 2, p, (m1 >> lane) & 1, m1);
+                    if (m2) record_hits_mask(J, at + 3, p, (m2 >> lane) & 1, m2);
+                    if (m3) record_hits_mask(J, at + 4, p, (m3 >> lane) & 1, m3);
                 }
-                idx += UNROLL;
             }
+            idx += kk;
             for (; kk < cnt; ++kk) {
                 step();
                 ++idx;
