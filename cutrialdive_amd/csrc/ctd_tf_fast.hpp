@@ -160,6 +160,32 @@ __device__ __forceinline__ uint64_t hit_base(uint64_t idx, uint32_t k)
         }                                                                                          \
     }
 
+// Eight indices per scalar branch: for the 4-5 instruction steps the mask
+// combining and loop control would otherwise keep the CU's scalar unit as
+// busy as its VALUs.  STEP8 sets the masks mk[0..7].
+#define CTD_UNROLLED_LOOP_MASKS8(STEP8, STEP1, MASK1)                                            \
+    {                                                                                              \
+        const uint32_t steps32 = (uint32_t)steps;                                                 \
+        uint32_t k = 0;                                                                            \
+        for (; k + 8 <= steps32; k += 8) {                                                         \
+            uint64_t mk[8];                                                                        \
+            STEP8;                                                                                 \
+            if ((mk[0] | mk[1] | mk[2] | mk[3]) | (mk[4] | mk[5] | mk[6] | mk[7])) {               \
+                const uint32_t lane = __lane_id();                                                 \
+                const uint64_t at = hit_base(idx, k);                                              \
+                for (int j = 0; j < 8; ++j)                                                        \
+                    if (mk[j]) record_hits_mask(J, at + 1 + j, p, (mk[j] >> lane) & 1, mk[j]);     \
+            }                                                                                      \
+        }                                                                                          \
+        idx += k;                                                                                  \
+        for (; k < steps32; ++k) {                                                                 \
+            STEP1;                                                                                 \
+            ++idx;                                                                                 \
+            const uint64_t m1 = MASK1;                                                             \
+            if (m1) record_hits_mask(J, idx, p, (m1 >> __lane_id()) & 1, m1);                      \
+        }                                                                                          \
+    }
+
 } // namespace ctd
 
 #include "ctd_tf_fp.hpp"

@@ -190,12 +190,15 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
         // the target class has the two representatives tgt and tgt -+ p
         const double tgt2 = (FAM == FAM_PROTH) ? F.p - 1.0 : 1.0 - F.p;
         record_hits(J, idx, p, x == tgt);
-        CTD_UNROLLED_LOOP_MASKS(
+        CTD_UNROLLED_LOOP_MASKS8(
             {
-                double y = x + x; m0 = __ballot(y == tgt) | __ballot(y == tgt2);
-                x = F.red(y + y); m1 = __ballot(x == tgt);
-                y = x + x; m2 = __ballot(y == tgt) | __ballot(y == tgt2);
-                x = F.red(y + y); m3 = __ballot(x == tgt);
+                _Pragma("unroll")
+                for (int j = 0; j < 8; j += 2) {
+                    const double y = x + x;
+                    mk[j] = __ballot(y == tgt) | __ballot(y == tgt2);
+                    x = F.red(y + y);
+                    mk[j + 1] = __ballot(x == tgt);
+                }
             },
             x = F.red(x + x), __ballot(x == tgt))
         return F.canon(FAM == FAM_PROTH ? x + 1.0 : x - 1.0, p);
@@ -207,12 +210,15 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
         record_hits(J, idx, p, x == 0.0);
         // every other index skips the reduction: y = x + c lies in (-p, p),
         // where the zero class is {0, -p, p}
-        CTD_UNROLLED_LOOP_MASKS(
+        CTD_UNROLLED_LOOP_MASKS8(
             {
-                double y = x + c; m0 = __ballot(y == 0.0) | __ballot(fabs(y) == F.p);
-                x = F.red(y + c); m1 = __ballot(x == 0.0);
-                y = x + c; m2 = __ballot(y == 0.0) | __ballot(fabs(y) == F.p);
-                x = F.red(y + c); m3 = __ballot(x == 0.0);
+                _Pragma("unroll")
+                for (int j = 0; j < 8; j += 2) {
+                    const double y = x + c;
+                    mk[j] = __ballot(y == 0.0) | __ballot(fabs(y) == F.p);
+                    x = F.red(y + c);
+                    mk[j + 1] = __ballot(x == 0.0);
+                }
             },
             x = F.red(x + c), __ballot(x == 0.0))
         return F.canon(x, p);
@@ -230,14 +236,17 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
             }
         }
         record_hits(J, idx, p, b == 0.0);
-        CTD_UNROLLED_LOOP(
+        CTD_UNROLLED_LOOP_MASKS8(
             {
-                a = F.red(a + b); h0 = a == 0.0;
-                b = F.red(a + b); h1 = b == 0.0;
-                a = F.red(a + b); h2 = a == 0.0;
-                b = F.red(a + b); h3 = b == 0.0;
+                _Pragma("unroll")
+                for (int j = 0; j < 8; j += 2) {
+                    a = F.red(a + b);
+                    mk[j] = __ballot(a == 0.0);
+                    b = F.red(a + b);
+                    mk[j + 1] = __ballot(b == 0.0);
+                }
             },
-            { const double t = F.red(a + b); a = b; b = t; }, b == 0.0)
+            { const double t = F.red(a + b); a = b; b = t; }, __ballot(b == 0.0))
         return F.canon(b, p);
     } else if constexpr (FAM == FAM_TRIBONACCI) {
         double a, b, c;
