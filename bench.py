@@ -62,14 +62,40 @@ def ops_table():
         return json.load(f)
 
 
-def ops_per_update(workload: str, seq: int, pmax: int):
+def loop_ops(t, fam: str, pmax: int, pmin: int) -> float:
+    """Loop VALU instructions per update of the form the residue kernel picks
+    for primes in [pmin, pmax] (ctd_tf_fast.hpp use_fp, ctd_tf_fp.hpp)."""
+    e = t["loop"][fam]
+    if pmax < (1 << 48):
+        if fam in ("fibonacci", "lucas") and pmax < (1 << 31):
+            return float(e["w32"])
+        if fam == "smarandache" and pmin < (1 << 35):
+            return float(e["w64"])
+        for bits, ops in e.get("fp64_by_bits", []):
+            if pmax < (1 << bits):
+                return float(ops)
+        return float(e["fp64"])
+    return float(e["w32"] if pmax < (1 << 31) else e["w64"])
+
+
+def window_ops(t, fam: str, lo: int, hi: int) -> float:
+    """Prime-count-weighted loop ops per update over the window [lo, hi)."""
+    cuts = sorted({lo, hi} | {1 << b for b in (31, 35, 37, 45, 48) if lo < (1 << b) < hi})
+    num = den = 0.0
+    for a, b in zip(cuts, cuts[1:]):
+        w = (b - a) / math.log(max(b, 16))
+        num += w * loop_ops(t, fam, b - 1, a)
+        den += w
+    return num / den
+
+
+def ops_per_update(workload: str, seq: int, lo: int, hi: int):
     """(VALU lane-ops per update, source) for a single-job workload."""
     t = ops_table()
     m = t["measured"].get(workload)
     if m:
         return float(m["valu_lane_ops_per_update"]), "rocprofv3 " + m["profile"]
-    fam = t["loop"][FAMILY_NAMES[seq]]
-    return float(fam["w32"] if pmax < (1 << 31) else fam["w64"]), "loop ISA count"
+    return window_ops(t, FAMILY_NAMES[seq], max(lo, 3), hi), "loop ISA count, prime-weighted"
 
 
 def mixed_ops(jobs, full_batch=True):
@@ -81,8 +107,7 @@ def mixed_ops(jobs, full_batch=True):
     num = den = 0.0
     for j in jobs:
         u = (j.f1 - j.f0) / math.log(j.f1) * (j.n1 - j.n0)
-        form = "fp64" if j.f1 <= (1 << 48) else "w64"
-        num += u * float(t["loop"][FAMILY_NAMES[j.seq_id]][form])
+        num += u * window_ops(t, FAMILY_NAMES[j.seq_id], j.f0, j.f1)
         den += u
     return num / den, "loop ISA count, update-weighted"
 
@@ -445,7 +470,7 @@ def run_single(args, torch, ctd, nat):
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     value = updates / dt
-    c_upd, c_src = ops_per_update(args.workload, seq, F1 - 1)
+    c_upd, c_src = ops_per_update(args.workload, seq, F0, F1)
     peak = eng.valu_probe(0, 4096) / 1e12
     avg_launch_s = tf_ms / 1e3 / max(launches, 1)
     achieved = updates / max(launches, 1) * c_upd / avg_launch_s / 1e12

@@ -60,9 +60,9 @@ struct JobArgs {
 // Closed forms of value_mod_2 for each family.  The reference propagates
 // them with next_value_mod_2 (trial_factoring.cuh:139-154); for every
 // built-in family that propagation reproduces value_mod_2(n) at each n
-// (mersenne.hpp:69-107, riesel_proth.hpp:199-213, riesel_proth_fixed_n.hpp:421-444,
-// smarandache.hpp:83-112, fibonacci.hpp:49-61, lucas.hpp:199-210,
-// tribonacci.hpp:372-383), so each index can be evaluated on its own.
+// (mersenne.hpp:70-107, riesel_proth.hpp:76-88, riesel_proth_fixed_n.hpp:119-140,
+// smarandache.hpp:85-112, fibonacci.hpp:50-61, lucas.hpp:41-51,
+// tribonacci.hpp:105-115), so each index can be evaluated on its own.
 __device__ __forceinline__ uint32_t value_mod_2(uint32_t fam, uint64_t param, uint64_t n)
 {
     switch (fam) {

@@ -186,6 +186,34 @@ __device__ __forceinline__ uint64_t hit_base(uint64_t idx, uint32_t k)
         }                                                                                          \
     }
 
+// Sixteen indices per scalar branch, for the stride forms whose VALU work per
+// index is one compare: 15 mask ORs and 3 loop instructions on the scalar
+// unit against 19-22 VALU instructions.  STEP16 sets the masks mk[0..15].
+#define CTD_UNROLLED_LOOP_MASKS16(STEP16, STEP1, MASK1)                                          \
+    {                                                                                              \
+        const uint32_t steps32 = (uint32_t)steps;                                                 \
+        uint32_t k = 0;                                                                            \
+        for (; k + 16 <= steps32; k += 16) {                                                       \
+            uint64_t mk[16];                                                                       \
+            STEP16;                                                                                \
+            const uint64_t lo8 = (mk[0] | mk[1] | mk[2] | mk[3]) | (mk[4] | mk[5] | mk[6] | mk[7]);   \
+            const uint64_t hi8 = (mk[8] | mk[9] | mk[10] | mk[11]) | (mk[12] | mk[13] | mk[14] | mk[15]); \
+            if (lo8 | hi8) {                                                                       \
+                const uint32_t lane = __lane_id();                                                 \
+                const uint64_t at = hit_base(idx, k);                                              \
+                for (int j = 0; j < 16; ++j)                                                       \
+                    if (mk[j]) record_hits_mask(J, at + 1 + j, p, (mk[j] >> lane) & 1, mk[j]);     \
+            }                                                                                      \
+        }                                                                                          \
+        idx += k;                                                                                  \
+        for (; k < steps32; ++k) {                                                                 \
+            STEP1;                                                                                 \
+            ++idx;                                                                                 \
+            const uint64_t m1 = MASK1;                                                             \
+            if (m1) record_hits_mask(J, idx, p, (m1 >> __lane_id()) & 1, m1);                      \
+        }                                                                                          \
+    }
+
 } // namespace ctd
 
 #include "ctd_tf_fp.hpp"
@@ -194,15 +222,18 @@ namespace ctd {
 
 // The FP64 form (ctd_tf_fp.hpp) takes a wave when the host allows it
 // (JOB_FP64: every prime of the batch below FP_PMAX) and it is the cheaper
-// form for the wave: the 32-bit integer steps stay cheaper for doubling,
-// fixed-n and Fibonacci/Lucas when every prime is below 2^31; Smarandache
-// needs every prime at least FP_SM_PMIN and the narrow multiplier bound.
+// form for the wave: the 32-bit integer steps stay cheaper for
+// Fibonacci/Lucas when every prime is below 2^31 (the stride forms of the
+// doubling and fixed-n families beat them at every size); Smarandache needs
+// every prime at least FP_SM_PMIN and the narrow multiplier bound.
 template <uint32_t FAM>
 __device__ __forceinline__ bool use_fp(const JobArgs &J, uint64_t p)
 {
     if (!(J.flags & JOB_FP64) || __ballot(p >= FP_PMAX) != 0) return false;
     if constexpr (FAM == FAM_PEANO) return false;
-    else if constexpr (FAM == FAM_TRIBONACCI) return true;
+    else if constexpr (FAM == FAM_TRIBONACCI || FAM == FAM_MERSENNE || FAM == FAM_PROTH || FAM == FAM_RIESEL ||
+                       FAM == FAM_PROTH_FIXED_N || FAM == FAM_RIESEL_FIXED_N)
+        return true; // the stride forms beat the 4-instruction 32-bit steps
     else if constexpr (FAM == FAM_SMARANDACHE)
         return (J.flags & JOB_SM_NARROW) && __ballot(p < FP_SM_PMIN) == 0;
     else return !wave_w32(p);

@@ -17,10 +17,26 @@
 //
 // Every residue class has one representative with |x| < p, so after a
 // reduction the zero test is x == 0 and the doubling families test x == +-1.
+//
+// Stride forms (doubling and fixed-n families).  When the reduction is exact
+// (round(v/p) correct, which holds when |v/p| 2^-53 < 1/(2p): the distance of
+// v/p from a half-integer is at least 1/(2p) for odd p) every class has the
+// single representative |x| <= (p-1)/2.  Then the S indices after the state
+// x_n are tested by S compares of x_n against precomputed targets, and the
+// state advances S indices at once:
+//   doubling  x_{n+j} = x_n 2^j:   hit at n+j  <=>  x_n == t 2^-j,
+//             x <- x 2^S - p round(x 2^S / p)  (3 FMAs: the product by 2^S
+//             is exact, and p round() is exact below 2^53);
+//   fixed n   x_{n+j} = x_n + j c: hit at n+j  <=>  x_n == -j c,
+//             x <- red(x + S c)  (4 instructions).
+// |x 2^S / p| < 2^(S-1), so the doubling stride is exact for S = 16 when
+// p < 2^37, S = 8 when p < 2^45 and S = 4 when p < 2^49.
+//
 // Per index this costs (VALU instructions, compare included):
 //
-//   doubling x = 2x           4   (integer W64 form: 7); every other index
-//   fixed n  x = x + c        4   (6)   skips the reduction and compares twice
+//   doubling x = 2x           1.19 (p < 2^37), 1.375 (p < 2^45), 1.75
+//                                  (integer W64 form: 7)
+//   fixed n  x = x + c        1.25 (6)
 //   fibonacci / lucas         5   (6)
 //   tribonacci                6   (11.25)
 //   smarandache r m + n + 1   9   (12.25), for 2^35 <= p
@@ -181,44 +197,71 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
     Fmod F;
     F.init(p);
     if constexpr (FAM == FAM_MERSENNE || FAM == FAM_PROTH || FAM == FAM_RIESEL) {
-        // x = c 2^n, hit when x == target (the representative of +-1 is +-1)
+        // x = c 2^n, hit when x == t (t = -1 for proth, +1 otherwise)
         double x = F.pow2(ns);
         if constexpr (FAM != FAM_MERSENNE) x = F.mul(F.of(J.param < p ? J.param : J.param % p), x);
         x = F.red(x);
         const double tgt = (FAM == FAM_PROTH) ? -1.0 : 1.0;
-        // every other index skips the reduction: y = 2x lies in (-p, p), where
-        // the target class has the two representatives tgt and tgt -+ p
-        const double tgt2 = (FAM == FAM_PROTH) ? F.p - 1.0 : 1.0 - F.p;
         record_hits(J, idx, p, x == tgt);
-        CTD_UNROLLED_LOOP_MASKS8(
-            {
-                _Pragma("unroll")
-                for (int j = 0; j < 8; j += 2) {
-                    const double y = x + x;
-                    mk[j] = __ballot(y == tgt) | __ballot(y == tgt2);
-                    x = F.red(y + y);
-                    mk[j + 1] = __ballot(x == tgt);
-                }
-            },
-            x = F.red(x + x), __ballot(x == tgt))
+        // T[j] = t 2^-(j+1), reduced; 2^-1 = (p+1)/2
+        const double inv2 = 0.5 * (F.p + 1.0);
+        double T[16];
+        T[0] = F.red(tgt * inv2);
+        _Pragma("unroll")
+        for (int j = 1; j < 16; ++j) T[j] = F.red(F.mul(T[j - 1], inv2));
+        const double mp = -F.p, pM = F.p * Fmod::MAGIC;
+        if (__ballot(p >= (1ull << 37)) == 0) {
+            const double ip = F.invp * 65536.0;
+            CTD_UNROLLED_LOOP_MASKS16(
+                {
+                    _Pragma("unroll")
+                    for (int j = 0; j < 16; ++j) mk[j] = __ballot(x == T[j]);
+                    x = fma(x, 65536.0, fma(fma(x, ip, Fmod::MAGIC), mp, pM));
+                },
+                x = F.red(x + x), __ballot(x == tgt))
+        } else if (__ballot(p >= (1ull << 45)) == 0) {
+            const double ip = F.invp * 256.0;
+            CTD_UNROLLED_LOOP_MASKS16(
+                {
+                    _Pragma("unroll")
+                    for (int h = 0; h < 16; h += 8) {
+                        _Pragma("unroll")
+                        for (int j = 0; j < 8; ++j) mk[h + j] = __ballot(x == T[j]);
+                        x = fma(x, 256.0, fma(fma(x, ip, Fmod::MAGIC), mp, pM));
+                    }
+                },
+                x = F.red(x + x), __ballot(x == tgt))
+        } else {
+            const double ip = F.invp * 16.0;
+            CTD_UNROLLED_LOOP_MASKS16(
+                {
+                    _Pragma("unroll")
+                    for (int h = 0; h < 16; h += 4) {
+                        _Pragma("unroll")
+                        for (int j = 0; j < 4; ++j) mk[h + j] = __ballot(x == T[j]);
+                        x = fma(x, 16.0, fma(fma(x, ip, Fmod::MAGIC), mp, pM));
+                    }
+                },
+                x = F.red(x + x), __ballot(x == tgt))
+        }
         return F.canon(FAM == FAM_PROTH ? x + 1.0 : x - 1.0, p);
     } else if constexpr (FAM == FAM_PROTH_FIXED_N || FAM == FAM_RIESEL_FIXED_N) {
-        // x = S(k) = k 2^n +- 1; step x += 2^n
-        const double c = F.pow2(J.param);
+        // x = S(k) = k 2^n +- 1; step x += c with c = 2^n; hit x == 0
+        const double c = F.red(F.pow2(J.param));
         const uint64_t k = ns < (1ull << 52) ? ns : ns % p;
         double x = F.red(F.mul(F.of(k), c) + (FAM == FAM_PROTH_FIXED_N ? 1.0 : -1.0));
         record_hits(J, idx, p, x == 0.0);
-        // every other index skips the reduction: y = x + c lies in (-p, p),
-        // where the zero class is {0, -p, p}
-        CTD_UNROLLED_LOOP_MASKS8(
+        // T[j] = -(j+1) c, reduced; the state advances by 16 c
+        double T[16];
+        T[0] = -c;
+        _Pragma("unroll")
+        for (int j = 1; j < 16; ++j) T[j] = F.red(T[j - 1] - c);
+        const double c16 = F.red(16.0 * c);
+        CTD_UNROLLED_LOOP_MASKS16(
             {
                 _Pragma("unroll")
-                for (int j = 0; j < 8; j += 2) {
-                    const double y = x + c;
-                    mk[j] = __ballot(y == 0.0) | __ballot(fabs(y) == F.p);
-                    x = F.red(y + c);
-                    mk[j + 1] = __ballot(x == 0.0);
-                }
+                for (int j = 0; j < 16; ++j) mk[j] = __ballot(x == T[j]);
+                x = F.red(x + c16);
             },
             x = F.red(x + c), __ballot(x == 0.0))
         return F.canon(x, p);
