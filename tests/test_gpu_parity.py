@@ -212,3 +212,42 @@ def test_small_batches_and_resume_accumulate(engine):
     assert res.factors == full
     assert last2 == last
     assert last1 == int(o.sieve((1 << 21) - 100, 1 << 21)[-1])
+
+
+# Windows for the stride forms of the doubling and fixed-n families: below
+# 2^37 (16 indices per reduction), below 2^45 (8), below 2^48 (4), and two
+# windows straddling 2^37 and 2^45 so one wave holds primes of both ranges.
+STRIDE_WINDOWS = [(1 << 36) + 24690, (1 << 37) - 2000, (1 << 44) + (1 << 40), (1 << 45) - 2000,
+                  (1 << 47) + (1 << 30)]
+# Offsets of the planted hit from n0: the check before the loop, every
+# position of a 16-index block, and the single-step tail (N = 600: blocks
+# cover offsets 1..592, the tail 593..599).
+STRIDE_OFFSETS = [0, 1, 2, 5, 15, 16, 17, 300, 592, 593, 599]
+
+
+@pytest.mark.parametrize("seq", [o.PROTH, o.RIESEL, o.PROTH_FIXED_N, o.RIESEL_FIXED_N])
+@pytest.mark.parametrize("w", STRIDE_WINDOWS)
+def test_stride_planted_hits(engine, seq, w):
+    """At primes this large a random window holds no hits, so each job is
+    built around one: p0 divides S at a chosen index.  Proth/Riesel:
+    k = -+2^-n (mod p0) makes p0 | k 2^n +- 1; fixed n: the index k* = -+2^-n
+    (mod p0) does."""
+    f0, f1 = w, w + 4096
+    primes = [int(x) for x in o.sieve(f0, f1)]
+    p0 = primes[len(primes) // 3]
+    N = 600
+    for off in STRIDE_OFFSETS:
+        if seq in (o.PROTH, o.RIESEL):
+            n_hit = 1000 + 7 * off
+            inv = pow(2, -n_hit, p0)
+            param = (p0 - inv) % p0 if seq == o.PROTH else inv
+            n0 = n_hit - off
+        else:
+            param = 777
+            inv = pow(2, -param, p0)
+            k_hit = (p0 - inv) % p0 if seq == o.PROTH_FIXED_N else inv
+            k_hit += p0 * (1 + off % 3)  # index values above p0 as well
+            n0 = k_hit - off
+        g, gc, _ = gpu_lists(engine, seq, param, n0, n0 + N, f0, f1)
+        assert p0 in g[off], (seq, w, off)
+        assert_same(g, gc, seq, param, n0, n0 + N, f0, f1)
