@@ -64,13 +64,12 @@ def ops_table():
 
 def loop_ops(t, fam: str, pmax: int, pmin: int) -> float:
     """Loop VALU instructions per update of the form the residue kernel picks
-    for primes in [pmin, pmax] (ctd_tf_fast.hpp use_fp, ctd_tf_fp.hpp)."""
+    for primes in [pmin, pmax] (ctd_tf_fast.hpp use_fp, ctd_tf_fp.hpp); the
+    Smarandache FP64 form also needs p > 8 n1, which every bench job meets."""
     e = t["loop"][fam]
     if pmax < (1 << 48):
         if fam in ("fibonacci", "lucas") and pmax < (1 << 31):
             return float(e["w32"])
-        if fam == "smarandache" and pmin < (1 << 35):
-            return float(e["w64"])
         for bits, ops in e.get("fp64_by_bits", []):
             if pmax < (1 << bits):
                 return float(ops)

@@ -225,7 +225,7 @@ namespace ctd {
 // form for the wave: the 32-bit integer steps stay cheaper for
 // Fibonacci/Lucas when every prime is below 2^31 (the stride forms of the
 // doubling and fixed-n families beat them at every size); Smarandache needs
-// every prime at least FP_SM_PMIN and the narrow multiplier bound.
+// every prime above 8 n1 and the narrow multiplier bound.
 template <uint32_t FAM>
 __device__ __forceinline__ bool use_fp(const JobArgs &J, uint64_t p)
 {
@@ -235,7 +235,7 @@ __device__ __forceinline__ bool use_fp(const JobArgs &J, uint64_t p)
                        FAM == FAM_PROTH_FIXED_N || FAM == FAM_RIESEL_FIXED_N)
         return true; // the stride forms beat the 4-instruction 32-bit steps
     else if constexpr (FAM == FAM_SMARANDACHE)
-        return (J.flags & JOB_SM_NARROW) && __ballot(p < FP_SM_PMIN) == 0;
+        return (J.flags & JOB_SM_NARROW) && __ballot(p <= 8 * J.n1) == 0;
     else return !wave_w32(p);
 }
 

@@ -39,7 +39,7 @@
 //   fixed n  x = x + c        1.25 (6)
 //   fibonacci / lucas         5   (6)
 //   tribonacci                6   (11.25)
-//   smarandache r m + n + 1   9   (12.25), for 2^35 <= p
+//   smarandache r m + n + 1   7   (12.25), for p > 8 n1
 //
 // and the per-prime set-up (powers, doubling formulas, Smarandache jump) runs
 // on 6-instruction multiplies instead of ~40-instruction Montgomery products.
@@ -50,7 +50,6 @@
 namespace ctd {
 
 constexpr uint64_t FP_PMAX = 1ull << 48;   // every prime of the wave must be below
-constexpr uint64_t FP_SM_PMIN = 1ull << 35; // smarandache: r m + lo + n + 1 stays below 0.75 p
 
 struct Fmod {
     double p, invp;
@@ -189,7 +188,7 @@ __device__ inline double sm_value_fp(const Fmod &F, uint64_t b, uint64_t n)
 }
 
 // The FP64 form of Fast<FAM>::run for a wave whose primes are all below
-// FP_PMAX (and, for smarandache, at least FP_SM_PMIN).  Returns the plain
+// FP_PMAX (and, for smarandache, above 8 n1).  Returns the plain
 // residue S(ne-1) mod p for the test hook.
 template <uint32_t FAM>
 __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t ne, uint64_t idx, uint64_t steps)
@@ -306,29 +305,48 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
             { const double t = F.red(a + b + c); a = b; b = c; c = t; }, a == 0.0)
         return F.canon(a, p);
     } else if constexpr (FAM == FAM_SMARANDACHE) {
-        // r = S(n); step r = r m + (n+1) with m = b^digits(n+1) < 2^21 (host
-        // flag JOB_SM_NARROW) and n+1 < 2^32: hi - p round(hi/p) is within
-        // 0.5 p, lo within 2^16, so |r| < 0.5 p + 2^33 <= 0.75 p for p >= 2^35
-        // and S(n) = 0 mod p  <=>  r == 0.
+        // r = S(n); step r = r m + c with m = b^digits(n+1) < 2^21 (host flag
+        // JOB_SM_NARROW) and c = n+1 < 2^32.  Write p = ph + pl with ph the top
+        // 32 significant bits of p and let k = round(r m / p), |k| < 2^21:
+        //   r' = (r m - k ph) + c - k pl.
+        // k ph is exact (at most 21 + 32 significant bits); r m - k ph =
+        // (r m - k p) + k pl lies below 2^53, so one fma computes it exactly;
+        // the rest is small.  |r m - k p| <= (1/2 + 2^-31) p, so |r'| <= 0.75 p
+        // whenever p > 8 n1 (checked per wave), and S(n) = 0 mod p <=> r == 0.
+        // c is wave-uniform and lives in scalar registers as the bit pattern
+        // of the double: inside one binade [2^e, 2^(e+1)) consecutive integers
+        // differ by 2^(52-e) in the pattern (and the carry into the exponent at
+        // 2^(e+1) still gives the exact value), so an integer add steps it and
+        // the VALU does 7 instructions per index, the compare included.
         const uint64_t bb = J.param;
         double r = sm_value_fp(F, bb, ns);
         record_hits(J, idx, p, r == 0.0);
+        const uint32_t L = 64 - __clzll(p);
+        const uint64_t phi = L > 32 ? p & ~((1ull << (L - 32)) - 1) : p;
+        const double ph = (double)phi, pl = (double)(p - phi);
         uint32_t d;
         uint64_t nxt;
         digits_pow(bb, ns + 1, d, nxt);
         double m = 1.0;
         for (uint32_t i = 0; i < d; ++i) m *= (double)bb;
-        double c = (double)(ns + 1); // the number appended by the next step (wave-uniform)
-        auto step = [&](void) {
-            const double hi = r * m;
-            const double lo = fma(r, m, -hi);
-            r = fma(-F.quot(hi), F.p, hi) + (lo + c);
-            c += 1.0;
-        };
-        uint64_t n = ns;
+        double mp = m * F.invp;
+        uint64_t n = ns; // index of the current state
         while (n + 1 < ne) {
-            const uint64_t seg_last = nxt - 2 < ne - 1 ? nxt - 2 : ne - 2; // last n of a step with this m
+            // steps n -> n+1 .. seg_last -> seg_last+1 share m = b^digits(n+1)
+            // (digits grow when n+1 reaches nxt) and the binade of c = n+1
+            const uint64_t c0 = n + 1;
+            const uint32_t e = 63 - __clzll(c0);
+            uint64_t seg_last = nxt - 2 < ne - 2 ? nxt - 2 : ne - 2;
+            if ((2ull << e) - 2 < seg_last) seg_last = (2ull << e) - 2;
             const uint32_t cnt = (uint32_t)(seg_last - n + 1); // below 2^32: n1 <= 2^32
+            const uint64_t delta = 1ull << (52 - e);
+            uint64_t cb = __double_as_longlong((double)c0);
+            auto step = [&](void) {
+                const double k = fma(r, mp, Fmod::MAGIC) - Fmod::MAGIC;
+                const double t = fma(r, m, -(k * ph));
+                r = fma(-k, pl, t + __longlong_as_double(cb));
+                cb += delta;
+            };
             uint32_t kk = 0;
             for (; kk + UNROLL <= cnt; kk += UNROLL) {
                 step(); const uint64_t m0 = __ballot(r == 0.0);
@@ -352,8 +370,9 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
                 if (mk) record_hits_mask(J, idx, p, (mk >> __lane_id()) & 1, mk);
             }
             n = seg_last + 1;
-            if (n + 1 < ne) { // the next appended number has one more digit
+            if (n + 1 == nxt) { // the next appended number has one more digit
                 m *= (double)bb;
+                mp = m * F.invp;
                 nxt = sat_mul(nxt, bb);
             }
         }
