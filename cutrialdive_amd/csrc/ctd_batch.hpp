@@ -173,6 +173,7 @@ void issue_tile(ctd_hip_engine *E, const JobPlan &P, BatchSlot &S, uint64_t blo,
     A.sp_mu = E->d_mu;
     A.nsp = E->nsp;
     A.nsmall = E->nsmall;
+    A.nwheel = E->nwheel;
     A.mode = SIEVE_DENSE;
     A.seg_counts = S.segcnt;
     A.out64 = S.primes;
@@ -215,6 +216,7 @@ LaunchShape tune_shape(ctd_hip_engine *E, const JobPlan &P, bool fast)
     A.sp_mu = E->d_mu;
     A.nsp = E->nsp;
     A.nsmall = E->nsmall;
+    A.nwheel = E->nwheel;
     A.mode = SIEVE_DENSE;
     A.seg_counts = S.segcnt;
     A.out64 = S.primes;

@@ -170,7 +170,7 @@ struct ctd_hip_engine {
     uint64_t base_lim = 0;
     uint32_t *d_sp = nullptr;
     uint64_t *d_mu = nullptr;
-    uint32_t nsp = 0, nsmall = 0;
+    uint32_t nsp = 0, nsmall = 0, nwheel = 0;
     // prime stream and scratch
     uint64_t *d_primes = nullptr;
     uint64_t primes_cap = 0;
@@ -248,7 +248,7 @@ constexpr uint64_t CHUNK_MAX = 1ull << 30;
 // Sieves [start, end) into an ordered list (u32 or u64) with the current
 // sieving numbers sp/mu.  Returns the count; allocates *out32 or *out64.
 uint64_t sieve_ordered(ctd_hip_engine *E, const uint32_t *sp, const uint64_t *mu, uint32_t nsp, uint32_t nsmall,
-                       uint64_t start, uint64_t end, uint32_t **out32, uint64_t **out64)
+                       uint32_t nwheel, uint64_t start, uint64_t end, uint32_t **out32, uint64_t **out64)
 {
     if (start >= end) return 0;
     const uint64_t lo0 = start & ~1ull;
@@ -263,6 +263,7 @@ uint64_t sieve_ordered(ctd_hip_engine *E, const uint32_t *sp, const uint64_t *mu
     A.sp_mu = mu;
     A.nsp = nsp;
     A.nsmall = nsmall;
+    A.nwheel = nwheel;
     A.mode = SIEVE_COUNT;
     A.seg_counts = E->d_segcnt;
     A.seg_off = E->d_segoff;
@@ -294,14 +295,24 @@ uint64_t sieve_ordered(ctd_hip_engine *E, const uint32_t *sp, const uint64_t *mu
     return total;
 }
 
-uint32_t count_small(ctd_hip_engine *E, const uint32_t *d_list, uint32_t n)
+// Entries of an ascending sieving list below `limit` (at most 4096 looked at).
+uint32_t count_below(ctd_hip_engine *E, const uint32_t *d_list, uint32_t n, uint32_t limit)
 {
     uint32_t k = std::min<uint32_t>(n, 4096);
     std::vector<uint32_t> h(k);
     if (k) HIPCHK(hipMemcpy(h.data(), d_list, k * sizeof(uint32_t), hipMemcpyDeviceToHost));
     uint32_t s = 0;
-    while (s < k && h[s] < SMALL_LIMIT) ++s;
+    while (s < k && h[s] < limit) ++s;
     return s;
+}
+
+// The small and wheel counts of a sieving list, checked against the sieve's
+// LDS phase table.
+void count_small(ctd_hip_engine *E, const uint32_t *d_list, uint32_t n, uint32_t &nsmall, uint32_t &nwheel)
+{
+    nsmall = count_below(E, d_list, n, SMALL_LIMIT);
+    nwheel = count_below(E, d_list, n, WHEEL_MAX + 1);
+    if (nsmall - nwheel > SMALL_MAX_COUNT) throw HipError{hipErrorInvalidValue, "sieving list: too many small numbers"};
 }
 
 void make_mu(ctd_hip_engine *E, const uint32_t *sp, uint64_t *mu, uint32_t n)
@@ -325,16 +336,17 @@ void build_base(ctd_hip_engine *E, uint64_t lim)
     hipLaunchKernelGGL(odd_numbers_kernel, dim3(1), dim3(128), 0, E->stream, l0, n0);
     make_mu(E, l0, m0, n0);
     uint32_t *l1 = nullptr;
-    uint32_t n1 = (uint32_t)sieve_ordered(E, l0, m0, n0, n0, 3, 65537, &l1, nullptr);
+    uint32_t n1 = (uint32_t)sieve_ordered(E, l0, m0, n0, n0, (WHEEL_MAX - 1) / 2, 3, 65537, &l1, nullptr);
     HIPCHK(hipFree(l0));
     HIPCHK(hipFree(m0));
     uint64_t *m1 = nullptr;
     HIPCHK(hipMalloc((void **)&m1, n1 * sizeof(uint64_t)));
     make_mu(E, l1, m1, n1);
-    uint32_t s1 = count_small(E, l1, n1);
+    uint32_t s1, w1;
+    count_small(E, l1, n1, s1, w1);
     // level 1: primes <= 65536 sieve [3, lim + 1)
     uint32_t *l2 = nullptr;
-    uint32_t n2 = (uint32_t)sieve_ordered(E, l1, m1, n1, s1, 3, lim + 1, &l2, nullptr);
+    uint32_t n2 = (uint32_t)sieve_ordered(E, l1, m1, n1, s1, w1, 3, lim + 1, &l2, nullptr);
     HIPCHK(hipFree(l1));
     HIPCHK(hipFree(m1));
     uint64_t *m2 = nullptr;
@@ -346,7 +358,7 @@ void build_base(ctd_hip_engine *E, uint64_t lim)
     E->d_sp = l2;
     E->d_mu = m2;
     E->nsp = n2;
-    E->nsmall = count_small(E, l2, n2);
+    count_small(E, l2, n2, E->nsmall, E->nwheel);
     E->base_lim = lim;
 }
 
@@ -484,6 +496,7 @@ int run_job(ctd_hip_engine *E, const ctd_job *job, const ctd_run_opts *opts, con
             A.sp_mu = E->d_mu;
             A.nsp = E->nsp;
             A.nsmall = E->nsmall;
+            A.nwheel = E->nwheel;
             A.mode = SIEVE_DENSE;
             A.seg_counts = E->d_segcnt;
             A.out64 = E->d_primes;
@@ -661,7 +674,7 @@ int ctd_hip_sieve(ctd_hip_engine *E, uint64_t start, uint64_t end, uint64_t *out
         uint64_t *d = nullptr;
         if (s < end) {
             build_base(E, isqrt_u64(end - 1));
-            n = sieve_ordered(E, E->d_sp, E->d_mu, E->nsp, E->nsmall, s, end, nullptr, &d);
+            n = sieve_ordered(E, E->d_sp, E->d_mu, E->nsp, E->nsmall, E->nwheel, s, end, nullptr, &d);
         }
         *count = n + n2;
         if (out && cap) {

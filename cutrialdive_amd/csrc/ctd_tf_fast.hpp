@@ -223,11 +223,12 @@ namespace ctd {
 // The FP64 form (ctd_tf_fp.hpp) takes a wave when the host allows it
 // (JOB_FP64: every prime of the batch below FP_PMAX) and it is the cheaper
 // form for the wave: the 32-bit integer steps stay cheaper for
-// Fibonacci/Lucas when every prime is below 2^31 (the stride forms of the
-// doubling and fixed-n families beat them at every size); Smarandache needs
-// every prime above 8 n1 and the narrow multiplier bound.
+// Fibonacci/Lucas when every prime is below 2^31, unless the work item is
+// long enough for the theta stride form (the stride forms of the doubling and
+// fixed-n families beat them at every size); Smarandache needs every prime
+// above 8 n1 and the narrow multiplier bound.
 template <uint32_t FAM>
-__device__ __forceinline__ bool use_fp(const JobArgs &J, uint64_t p)
+__device__ __forceinline__ bool use_fp(const JobArgs &J, uint64_t p, uint64_t steps)
 {
     if (!(J.flags & JOB_FP64) || __ballot(p >= FP_PMAX) != 0) return false;
     if constexpr (FAM == FAM_PEANO) return false;
@@ -236,6 +237,8 @@ __device__ __forceinline__ bool use_fp(const JobArgs &J, uint64_t p)
         return true; // the stride forms beat the 4-instruction 32-bit steps
     else if constexpr (FAM == FAM_SMARANDACHE)
         return (J.flags & JOB_SM_NARROW) && __ballot(p <= 8 * J.n1) == 0;
+    else if constexpr (FAM == FAM_FIBONACCI || FAM == FAM_LUCAS)
+        return !wave_w32(p) || (J.n0 != 0 && steps >= THETA_MIN_STEPS && __ballot(p == 5) == 0);
     else return !wave_w32(p);
 }
 
@@ -250,7 +253,7 @@ struct Fast {
         auto emit = [&](uint64_t v) {
             if (J.final_res) J.final_res[pi] = v;
         };
-        if (use_fp<FAM>(J, p)) {
+        if (use_fp<FAM>(J, p, ne - ns - 1)) {
             emit(run_fp<FAM>(J, p, ns, ne, ns - J.n0, ne - ns - 1));
             return;
         }

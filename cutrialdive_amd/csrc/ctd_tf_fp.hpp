@@ -187,6 +187,55 @@ __device__ inline double sm_value_fp(const Fmod &F, uint64_t b, uint64_t n)
     return S;
 }
 
+// Z_p[phi] / (phi^2 - phi - 1), elements u + v phi with reduced coordinates.
+// theta = phi / psi = -(1 + phi) (psi = 1 - phi, phi psi = -1), so that
+// F(n) (phi - psi) = psi^n (theta^n - 1) and L(n) = psi^n (theta^n + 1).  For
+// p != 2, 5 both phi - psi (its square is 5) and psi are units, hence
+//   F(n) = 0 mod p  <=>  theta^n = 1,   L(n) = 0 mod p  <=>  theta^n = -1.
+struct Rphi {
+    double u, v;
+};
+
+// x theta = -(u + v) - (u + 2v) phi
+__device__ __forceinline__ Rphi rphi_mul_theta(const Fmod &F, Rphi x)
+{
+    const double s = x.u + x.v;
+    return {F.red(-s), F.red(-s - x.v)};
+}
+// x theta^-1 = (v - 2u) + (u - v) phi   (theta^-1 = -2 + phi)
+__device__ __forceinline__ Rphi rphi_mul_theta_inv(const Fmod &F, Rphi x)
+{
+    return {F.red(x.v - 2.0 * x.u), F.red(x.u - x.v)};
+}
+// x y = (xu yu + xv yv) + (xu yv + xv yu + xv yv) phi
+__device__ __forceinline__ Rphi rphi_mul(const Fmod &F, Rphi x, Rphi y)
+{
+    const double a = F.mul(x.u, y.u), b = F.mul(x.v, y.v);
+    const double c = F.mul(F.red(x.u + x.v), F.red(y.u + y.v)); // (xu + xv)(yu + yv)
+    return {F.red(a + b), F.red(c - a)};
+}
+// x times the constant g = (gu, gv), with gw = gu + gv reduced
+__device__ __forceinline__ Rphi rphi_mul_const(const Fmod &F, Rphi x, double gu, double gv, double gw)
+{
+    return {F.red(F.mul(x.u, gu) + F.mul(x.v, gv)), F.red(F.mul(x.u, gv) + F.mul(x.v, gw))};
+}
+// theta^e
+__device__ inline Rphi rphi_theta_pow(const Fmod &F, uint64_t e)
+{
+    Rphi x = {1.0, 0.0};
+    if (e == 0) return x;
+    for (int b = 63 - __clzll(e); b >= 0; --b) {
+        x = rphi_mul(F, x, x);
+        if ((e >> b) & 1) x = rphi_mul_theta(F, x);
+    }
+    return x;
+}
+
+// Indices per reduction of the theta stride form, and the shortest work item
+// that takes it (its set-up is about 40 ring products).
+constexpr int THETA_STRIDE = 32;
+constexpr uint64_t THETA_MIN_STEPS = 1024;
+
 // The FP64 form of Fast<FAM>::run for a wave whose primes are all below
 // FP_PMAX (and, for smarandache, above 8 n1).  Returns the plain
 // residue S(ne-1) mod p for the test hook.
@@ -265,6 +314,59 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
             x = F.red(x + c), __ballot(x == 0.0))
         return F.canon(x, p);
     } else if constexpr (FAM == FAM_FIBONACCI || FAM == FAM_LUCAS) {
+        if (J.n0 != 0 && steps >= THETA_MIN_STEPS && __ballot(p == 5) == 0) {
+            // theta stride form: x = theta^n, hit when x == eps (eps = 1 for
+            // Fibonacci, -1 for Lucas).  The THETA_STRIDE indices after x_n
+            // are hits when x_n == eps theta^-j.  Every power of theta has
+            // norm u^2 + uv - v^2 = 1, so for a given v at most two u qualify
+            // (u and -v - u): the loop compares the v coordinate only, one
+            // compare per index, and a match is confirmed on both coordinates
+            // before it is recorded.  x advances THETA_STRIDE indices by one
+            // ring product with theta^THETA_STRIDE.
+            const double eps = FAM == FAM_FIBONACCI ? 1.0 : -1.0;
+            Rphi x = rphi_theta_pow(F, ns);
+            record_hits(J, idx, p, x.u == eps && x.v == 0.0);
+            double Tv[THETA_STRIDE];
+            {
+                Rphi t = {eps, 0.0};
+                _Pragma("unroll")
+                for (int j = 0; j < THETA_STRIDE; ++j) {
+                    t = rphi_mul_theta_inv(F, t);
+                    Tv[j] = t.v;
+                }
+            }
+            Rphi g = {-1.0, -1.0}; // theta^THETA_STRIDE by squaring
+            for (int j = 1; j < THETA_STRIDE; j <<= 1) g = rphi_mul(F, g, g);
+            const double gw = F.red(g.u + g.v);
+            const uint32_t steps32 = (uint32_t)steps;
+            uint32_t k = 0;
+            for (; k + THETA_STRIDE <= steps32; k += THETA_STRIDE) {
+                uint64_t any = 0;
+                _Pragma("unroll")
+                for (int j = 0; j < THETA_STRIDE; ++j) any |= __ballot(x.v == Tv[j]);
+                if (any) {
+                    // confirm each match: y = x theta^(j+1) must equal eps
+                    const uint64_t at = hit_base(idx, k);
+                    Rphi y = x;
+                    for (int j = 0; j < THETA_STRIDE; ++j) {
+                        y = rphi_mul_theta(F, y);
+                        if (__ballot(x.v == Tv[j])) record_hits(J, at + 1 + j, p, y.u == eps && y.v == 0.0);
+                    }
+                }
+                x = rphi_mul_const(F, x, g.u, g.v, gw);
+            }
+            idx += k;
+            for (; k < steps32; ++k) {
+                x = rphi_mul_theta(F, x);
+                ++idx;
+                record_hits(J, idx, p, x.u == eps && x.v == 0.0);
+            }
+            // S(ne-1) for the test hook, by the doubling formulas
+            double fa, fb;
+            fib_pair_fp(F, ne - 1, fa, fb); // (F(ne-1), F(ne))
+            if constexpr (FAM == FAM_FIBONACCI) return F.canon(fa, p);
+            else return F.canon(F.red(fb + fb - fa), p); // L(n) = 2F(n+1) - F(n)
+        }
         double a = 0.0, b = 0.0; // (S(n-1), S(n)); n0 = 0 keeps {0, 0} (fibonacci.hpp:68-71)
         if (J.n0 != 0) {
             double fa, fb;
