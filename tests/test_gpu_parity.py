@@ -251,3 +251,40 @@ def test_stride_planted_hits(engine, seq, w):
         g, gc, _ = gpu_lists(engine, seq, param, n0, n0 + N, f0, f1)
         assert p0 in g[off], (seq, w, off)
         assert_same(g, gc, seq, param, n0, n0 + N, f0, f1)
+
+
+def _fib_factor_pairs():
+    import json
+    import os
+
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "fib_large_factors.json")
+    with open(path) as f:
+        return [tuple(x) for x in json.load(f)["pairs"]]
+
+
+@pytest.mark.parametrize("m,q", _fib_factor_pairs())
+def test_theta_form_fibonacci_lucas_large_primes(engine, m, q):
+    """The theta stride form (work items of at least 1024 indices) at primes
+    2^37..2^47, around known factors q | F(m): q divides F at every multiple
+    of its rank of apparition, and L at the odd multiples of half of it."""
+    f0, f1 = q - 3000, q + 3000
+    n0 = max(1, m - 600)
+    g, gc, _ = gpu_lists(engine, o.FIBONACCI, 0, n0, n0 + 2000, f0, f1)
+    assert q in g[m - n0]
+    assert_same(g, gc, o.FIBONACCI, 0, n0, n0 + 2000, f0, f1)
+    n0 = max(1, m // 2 - 500)
+    g, gc, _ = gpu_lists(engine, o.LUCAS, 0, n0, n0 + 2000, f0, f1)
+    assert_same(g, gc, o.LUCAS, 0, n0, n0 + 2000, f0, f1)
+
+
+@pytest.mark.parametrize("seq", [o.FIBONACCI, o.LUCAS])
+@pytest.mark.parametrize("n0,n1,f0,f1,chunk", [
+    (1, 5000, 0, 1 << 18, 0),               # small primes (hits everywhere), 5 in the window
+    (3, 4100, 1 << 30, (1 << 30) + (1 << 16), 0),
+    (40000, 43000, (1 << 42) + (1 << 30), (1 << 42) + (1 << 30) + (1 << 14), 0),
+    (7, 20000, 3, 1 << 16, 3000),           # chunks start mid-range: theta^n0 jump-ahead
+])
+def test_theta_form_windows(engine, seq, n0, n1, f0, f1, chunk):
+    kw = {"chunk_length": chunk} if chunk else {}
+    g, gc, _ = gpu_lists(engine, seq, 0, n0, n1, f0, f1, **kw)
+    assert_same(g, gc, seq, 0, n0, n1, f0, f1)
