@@ -312,3 +312,20 @@ def test_tf_threads_equivalent():
     a = o.trial_factor(o.PROTH, 1279, 0, 300, 0, 1 << 20, threads=1)
     b = o.trial_factor(o.PROTH, 1279, 0, 300, 0, 1 << 20, threads=8)
     assert a == b
+
+
+def test_fib_large_factor_fixture():
+    """tests/golden/fib_large_factors.json: each q divides F(m) (Python
+    integers), and the oracle lists q at index m."""
+    import json
+
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "fib_large_factors.json")) as f:
+        pairs = json.load(f)["pairs"]
+    assert len(pairs) >= 10
+    F = [0, 1]
+    for _ in range(2, 1500):
+        F.append(F[-1] + F[-2])
+    for m, q in pairs[:6]:
+        assert F[m] % q == 0
+        r = o.trial_factor(o.FIBONACCI, 0, m - 2, m + 3, q - 100, q + 100)
+        assert q in r[2]
