@@ -17,6 +17,9 @@ device batch of 2^31 integers of the prime window:
   cfg3a  Sm16, n in [1, 2000), primes in [2^40, 2^40 + 2^36)
   cfg3b  Proth k=3, n in [1, 10^5), primes in [2^40, 2^40 + 2^32)
 
+Calibration (--workload copy): one 256 MiB device-to-device copy, the
+reference kernel for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE to bytes.
+
 Serving workload (--workload serve): a Poisson stream of jobs drawn from the
 same generator, with 2^24-wide prime windows so that 10^4 jobs take about a
 minute, submitted to the serving form (ctd_hip_server_*) at offered loads of
@@ -62,12 +65,15 @@ def ops_table():
         return json.load(f)
 
 
-def loop_ops(t, fam: str, pmax: int, pmin: int) -> float:
+def loop_ops(t, fam: str, pmax: int, pmin: int, n_items: int = 0) -> float:
     """Loop VALU instructions per update of the form the residue kernel picks
-    for primes in [pmin, pmax] (ctd_tf_fast.hpp use_fp, ctd_tf_fp.hpp); the
-    Smarandache FP64 form also needs p > 8 n1, which every bench job meets."""
+    for primes in [pmin, pmax] and work items of n_items indices
+    (ctd_tf_fast.hpp use_fp, ctd_tf_fp.hpp); the Smarandache FP64 form also
+    needs p > 8 n1, which every bench job meets."""
     e = t["loop"][fam]
     if pmax < (1 << 48):
+        if fam in ("fibonacci", "lucas") and n_items >= 1024:
+            return float(e["fp64_theta"])
         if fam in ("fibonacci", "lucas") and pmax < (1 << 31):
             return float(e["w32"])
         for bits, ops in e.get("fp64_by_bits", []):
@@ -77,24 +83,24 @@ def loop_ops(t, fam: str, pmax: int, pmin: int) -> float:
     return float(e["w32"] if pmax < (1 << 31) else e["w64"])
 
 
-def window_ops(t, fam: str, lo: int, hi: int) -> float:
+def window_ops(t, fam: str, lo: int, hi: int, n_items: int = 0) -> float:
     """Prime-count-weighted loop ops per update over the window [lo, hi)."""
     cuts = sorted({lo, hi} | {1 << b for b in (31, 35, 37, 45, 48) if lo < (1 << b) < hi})
     num = den = 0.0
     for a, b in zip(cuts, cuts[1:]):
         w = (b - a) / math.log(max(b, 16))
-        num += w * loop_ops(t, fam, b - 1, a)
+        num += w * loop_ops(t, fam, b - 1, a, n_items)
         den += w
     return num / den
 
 
-def ops_per_update(workload: str, seq: int, lo: int, hi: int):
+def ops_per_update(workload: str, seq: int, lo: int, hi: int, n_items: int):
     """(VALU lane-ops per update, source) for a single-job workload."""
     t = ops_table()
     m = t["measured"].get(workload)
     if m:
         return float(m["valu_lane_ops_per_update"]), "rocprofv3 " + m["profile"]
-    return window_ops(t, FAMILY_NAMES[seq], max(lo, 3), hi), "loop ISA count, prime-weighted"
+    return window_ops(t, FAMILY_NAMES[seq], max(lo, 3), hi, n_items), "loop ISA count, prime-weighted"
 
 
 def mixed_ops(jobs, full_batch=True):
@@ -106,9 +112,15 @@ def mixed_ops(jobs, full_batch=True):
     num = den = 0.0
     for j in jobs:
         u = (j.f1 - j.f0) / math.log(j.f1) * (j.n1 - j.n0)
-        num += u * window_ops(t, FAMILY_NAMES[j.seq_id], j.f0, j.f1)
+        num += u * window_ops(t, FAMILY_NAMES[j.seq_id], j.f0, j.f1, j.n1 - j.n0)
         den += u
     return num / den, "loop ISA count, update-weighted"
+
+
+def probe_peak(eng) -> float:
+    """The v_add_u32 lane-op peak (T/s): best of three 16384-trip probe runs
+    (single short runs vary with the clock the previous work left)."""
+    return max(eng.valu_probe(0, 16384) for _ in range(3)) / 1e12
 
 
 def cpu_sample(jobs, sample_len, threads):
@@ -221,7 +233,7 @@ def run_mixed(args, torch, ctd, nat):
         print(f"# plain form {pdt:.1f} s, identical factors: {match}", file=sys.stderr, flush=True)
 
     c_upd, c_src = mixed_ops(jobs, full_batch=args.family is None and args.jobs == 256)
-    peak = eng.valu_probe(0, 4096) / 1e12
+    peak = probe_peak(eng)
     per_launch_s = bms / 1e3 / max(graphs, 1)
     achieved = upd / max(graphs, 1) * c_upd / per_launch_s / 1e12
     traffic = 16.0 * primes / max(graphs, 1)  # sieve writes 8 B per prime, the residue kernel reads 8 B
@@ -264,7 +276,7 @@ def run_mixed(args, torch, ctd, nat):
             "kernel": "batch graph (sieve_kernel + tf_fast_kernel<family> per tile)",
             "ops_per_update": c_upd,
             "ops_per_update_source": c_src,
-            "peak_source": "v_add_u32 independent-chain probe, measured in this run",
+            "peak_source": "v_add_u32 independent-chain probe, best of 3 runs in this run",
             "nominal_peak": 39.3,
             "avg_launch_ms": per_launch_s * 1e3,
             "hbm": {"achieved_GBps": traffic / per_launch_s / 1e9, "peak_GBps": HBM_PEAK_GBPS,
@@ -389,9 +401,31 @@ def run_serve(args, torch, ctd, nat):
         pc, pf, _ = eng.run_job(c.seq_id, c.param, c.n0, c.n1, c.f0, c.f1, mx)
         plain.append(buffer_digest(pc, pf, mx))
     plain_s = time.perf_counter() - t0
+    served = [r["digests"] for r in runs]
     for r in runs:
         r["factors_identical_to_plain"] = r.pop("digests") == plain
     print(f"# plain form check {plain_s:.1f} s", file=sys.stderr, flush=True)
+
+    # CPU oracle on the whole windows of the jobs with the least work: their
+    # served buffers (every load) must equal the oracle's
+    cpu_check = None
+    if args.serve_cpu_jobs and not args.no_cpu:
+        from oracle import oracle as o
+
+        work = sorted(range(len(specs)), key=lambda k: (specs[k].n1 - specs[k].n0) * (specs[k].f1 - specs[k].f0))
+        pick = work[:args.serve_cpu_jobs]
+        threads = min(os.cpu_count() or 1, 16)
+        t0 = time.perf_counter()
+        ok = True
+        for k in pick:
+            j = specs[k]
+            rc, rf = o.trial_factor_raw(j.seq_id, j.param, j.n0, j.n1, j.f0, j.f1, mx, threads)
+            d = buffer_digest(rc, rf, mx)
+            ok = ok and all(sv[k] == d for sv in served)
+        cpu_check = {"jobs": len(pick), "seconds": time.perf_counter() - t0, "threads": threads,
+                     "served_equal_oracle": bool(ok),
+                     "selection": "the jobs of least work (indices x window), whole windows"}
+        print(f"# CPU oracle check of {len(pick)} served jobs: {ok}", file=sys.stderr, flush=True)
 
     top = runs[-1]
     result = {
@@ -421,9 +455,27 @@ def run_serve(args, torch, ctd, nat):
         },
         "serving": runs,
         "plain_check_seconds": plain_s,
+        "cpu_check": cpu_check,
     }
     print(json.dumps(result), flush=True)
     eng.close()
+    return 0
+
+
+def run_copy(args, torch, ctd, nat):
+    """HBM counter calibration: one device-to-device copy of a known size
+    (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this run give the factors
+    that scale the counters of the engine's kernels to bytes)."""
+    n = 256 << 20
+    src = torch.ones(n, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    print(json.dumps({"workload": "copy", "bytes_read": n, "bytes_written": n, "seconds": dt,
+                      "GBps": 2 * n / dt / 1e9}), flush=True)
     return 0
 
 
@@ -469,8 +521,8 @@ def run_single(args, torch, ctd, nat):
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     value = updates / dt
-    c_upd, c_src = ops_per_update(args.workload, seq, F0, F1)
-    peak = eng.valu_probe(0, 4096) / 1e12
+    c_upd, c_src = ops_per_update(args.workload, seq, F0, F1, N)
+    peak = probe_peak(eng)
     avg_launch_s = tf_ms / 1e3 / max(launches, 1)
     achieved = updates / max(launches, 1) * c_upd / avg_launch_s / 1e12
     traffic = 8.0 * primes / max(launches, 1)
@@ -501,7 +553,7 @@ def run_single(args, torch, ctd, nat):
             "kernel": f"tf_fast_kernel<{FAMILY_NAMES[seq]}>",
             "ops_per_update": c_upd,
             "ops_per_update_source": c_src,
-            "peak_source": "v_add_u32 independent-chain probe, measured in this run",
+            "peak_source": "v_add_u32 independent-chain probe, best of 3 runs in this run",
             "nominal_peak": 39.3,
             "avg_launch_ms": avg_launch_s * 1e3,
             "hbm": {"achieved_GBps": traffic / avg_launch_s / 1e9, "peak_GBps": HBM_PEAK_GBPS,
@@ -545,11 +597,12 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (mixed: 1, single-job workloads: 30)")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="mixed", choices=["mixed", "serve"] + sorted(SINGLE))
+    ap.add_argument("--workload", default="mixed", choices=["mixed", "serve", "copy"] + sorted(SINGLE))
     ap.add_argument("--serve-jobs", type=int, default=10000, help="serve: jobs per offered load")
     ap.add_argument("--loads", default="0.5,0.7,0.9", help="serve: offered loads (fractions of capacity)")
     ap.add_argument("--capacity-jobs", type=int, default=2000, help="serve: jobs of the capacity measurement")
     ap.add_argument("--serve-streams", type=int, default=4, help="serve: streams (resident prime slots)")
+    ap.add_argument("--serve-cpu-jobs", type=int, default=12, help="serve: served jobs checked against the oracle")
     ap.add_argument("--jobs", type=int, default=256, help="mixed: jobs drawn from the seeded generator")
     ap.add_argument("--tile-bits", type=int, default=0, help="mixed: log2 of the tile length (0 = engine default)")
     ap.add_argument("--no-autotune", action="store_true")
@@ -574,6 +627,8 @@ def main() -> int:
 
     if args.workload == "serve":
         return run_serve(args, torch, ctd, nat)
+    if args.workload == "copy":
+        return run_copy(args, torch, ctd, nat)
     if args.workload == "mixed":
         if args.steps is None:
             args.steps = 1

@@ -210,6 +210,12 @@ PROBE_OPS = [
     "v_fma_f64",
     "v_mul_f64",
     "v_max_f64",
+    "v_cmp_eq_u32",
+    "v_cmp_eq_f64",
+    "v_cmp_eq_u64",
+    "v_xor_b32",
+    "v_sub_u32",
+    "v_add_f32",
 ]
 
 _lib = None

@@ -971,7 +971,8 @@ int ctd_hip_valu_probe(ctd_hip_engine *E, int op, uint32_t iters, double *rate)
     case K: hipLaunchKernelGGL(valu_probe_kernel<K>, dim3(blocks), dim3(threads), 0, E->stream, d, it); break;
                 CTD_PROBE_CASE(0) CTD_PROBE_CASE(1) CTD_PROBE_CASE(2) CTD_PROBE_CASE(3) CTD_PROBE_CASE(4)
                 CTD_PROBE_CASE(5) CTD_PROBE_CASE(6) CTD_PROBE_CASE(7) CTD_PROBE_CASE(8) CTD_PROBE_CASE(9)
-                CTD_PROBE_CASE(10) CTD_PROBE_CASE(11) CTD_PROBE_CASE(12) CTD_PROBE_CASE(13)
+                CTD_PROBE_CASE(10) CTD_PROBE_CASE(11) CTD_PROBE_CASE(12) CTD_PROBE_CASE(13) CTD_PROBE_CASE(14)
+                CTD_PROBE_CASE(15) CTD_PROBE_CASE(16) CTD_PROBE_CASE(17) CTD_PROBE_CASE(18) CTD_PROBE_CASE(19)
 #undef CTD_PROBE_CASE
             }
         };

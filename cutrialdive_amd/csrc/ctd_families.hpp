@@ -17,7 +17,7 @@
 // be split into chunks that lanes process independently.  The values are the
 // exact residues, hence the hits equal the reference's zero residues, with one
 // deliberate exception kept for bit-exactness: fibonacci.hpp:68-71 starts
-// index 0 from {F(-1), F(0)} = {0, 0} (lucas.hpp:214-225 inherits it), which
+// index 0 from {F(-1), F(0)} = {0, 0} (lucas.hpp:54-66 inherits it), which
 // makes every later residue 0; a job starting at n0 = 0 reproduces that.
 #pragma once
 
@@ -107,7 +107,7 @@ __device__ inline void fib_pair(const Mont &M, uint64_t m, uint64_t &fa, uint64_
 }
 
 // (T(n), T(n+1), T(n+2)) by the doubling identities of OEIS A000073
-// (the ones tribonacci.hpp:318-330 also uses).
+// (the ones tribonacci.hpp:64-102 also uses).
 __device__ inline void trib_triple(const Mont &M, uint64_t n, uint64_t &ta, uint64_t &tb, uint64_t &tc)
 {
     const uint64_t p = M.p;

@@ -1,8 +1,8 @@
 """Python host-side mirror of cutrialdive's trial-factoring interface.
 
 Names follow the reference: num_seq_id / num_seq_spec (num_seq_id.hpp:16-27,
-num_seq_spec.hpp:19-26), the parameter parsing of dispatch_num_seq
-(num_seq_dispatch.hpp:266-315), trial_factoring_options
+num_seq_spec.hpp:16-22), the parameter parsing of dispatch_num_seq
+(num_seq_dispatch.hpp:119-168), trial_factoring_options
 (trial_factoring_options.hpp:25-39), trial_factor (trial_factoring.hpp:20-26)
 and the "<index> - <factors>" writer (factoring_results.hpp:183-201).
 Every computation goes through libctd_hip.so; nothing here factors anything.
@@ -75,7 +75,7 @@ def _parse_u64(s: str) -> Optional[int]:
 
 
 def spec_param(spec: NumSeqSpec) -> int:
-    """The numeric parameter dispatch_num_seq parses (num_seq_dispatch.hpp:266-315)."""
+    """The numeric parameter dispatch_num_seq parses (num_seq_dispatch.hpp:119-168)."""
     sid = spec.seq_id
     if sid in (NumSeqId.proth, NumSeqId.riesel):
         k = _parse_u64(spec.seq_params)

@@ -2,8 +2,8 @@
 //
 // Keeps cutrialdive's host-side API so that callers of the reference compile
 // against it unchanged:
-//   * number_sequence concepts          (number_sequence.hpp:19-58, number_sequence_detail.hpp:44-334)
-//   * num_seq_id / num_seq_spec         (num_seq_id.hpp:16-27, num_seq_spec.hpp:19-26)
+//   * number_sequence concepts          (number_sequence.hpp:19-58, number_sequence_detail.hpp:44-274)
+//   * num_seq_id / num_seq_spec         (num_seq_id.hpp:16-27, num_seq_spec.hpp:16-22)
 //   * dispatch_num_seq                  (num_seq_dispatch.hpp:119-168)
 //   * trial_factoring_options           (trial_factoring_options.hpp:25-39)
 //   * factors_buffer / factoring_results (factors_buffer.hpp:33-96, factoring_results.hpp:146-201)

@@ -23,7 +23,7 @@ logger &log()
     return l;
 }
 
-// num_seq_id_from_string / to_string (num_seq_id.cpp:25-43, num_seq_id.hpp:52-72)
+// num_seq_id_from_string / to_string (num_seq_id.cpp:25-43, num_seq_id.hpp:42-57)
 num_seq_id num_seq_id_from_string(std::string const &s)
 {
     static const std::unordered_map<std::string, num_seq_id> m{

@@ -181,7 +181,7 @@ struct mersenne {
     static std::ostream &print_expression(index_type n, std::ostream &out) { return out << "2^" << n << "-1"; }
 };
 
-// ---- Proth / Riesel k 2^n +- 1 (riesel_proth.hpp:146-304)
+// ---- Proth / Riesel k 2^n +- 1 (riesel_proth.hpp:23-179)
 enum class plus_or_minus { plus, minus };
 
 template <plus_or_minus PM>
@@ -240,7 +240,7 @@ private:
 using proth = riesel_proth<plus_or_minus::plus>;
 using riesel = riesel_proth<plus_or_minus::minus>;
 
-// ---- fixed-n Proth / Riesel k 2^n +- 1 indexed by k (riesel_proth_fixed_n.hpp:322-483)
+// ---- fixed-n Proth / Riesel k 2^n +- 1 indexed by k (riesel_proth_fixed_n.hpp:20-181)
 template <plus_or_minus PM>
 class riesel_proth_fixed_n_math {
 public:
@@ -446,7 +446,7 @@ public:
     std::ostream &print_value(index_type n, std::ostream &out) const { return out << value(n); }
 };
 
-// ---- Lucas (lucas.hpp:174-237); L(0) = 2, started from the Fibonacci state
+// ---- Lucas (lucas.hpp:15-79); L(0) = 2, started from the Fibonacci state
 class lucas {
 public:
     using index_type = uint64_t;
@@ -474,7 +474,7 @@ public:
     std::ostream &print_value(index_type n, std::ostream &out) const { return out << value(n); }
 };
 
-// ---- Tribonacci T(0) = T(1) = 0, T(2) = 1 (tribonacci.hpp:287-409)
+// ---- Tribonacci T(0) = T(1) = 0, T(2) = 1 (tribonacci.hpp:19-143)
 class tribonacci {
 public:
     struct state {

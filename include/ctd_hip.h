@@ -48,7 +48,7 @@ enum {
 
 /* One trial-factoring job: the fields of trial_factoring_options
  * (trial_factoring_options.hpp:25-39) plus the parsed sequence parameter of
- * num_seq_spec (num_seq_dispatch.hpp:266-315). */
+ * num_seq_spec (num_seq_dispatch.hpp:119-168). */
 typedef struct ctd_job {
     uint32_t seq_id;        /* CTD_SEQ_* */
     uint32_t max_factors;   /* max_factors_per_number (>= 1) */
@@ -72,7 +72,7 @@ typedef struct ctd_callbacks {
 } ctd_callbacks;
 
 /* Launch shape overrides: --grid-size / --block-size (command_line.cpp:148-164)
- * and --segment-length (trial_factoring_options.cpp:597).  Zero = autotuned. */
+ * and --segment-length (trial_factoring_options.cpp:31).  Zero = autotuned. */
 typedef struct ctd_run_opts {
     uint64_t batch_length;  /* integers of the prime range per device batch */
     int32_t grid_size;
@@ -249,7 +249,9 @@ int ctd_hip_residues(ctd_hip_engine *eng, const ctd_job *job, const uint64_t *pr
  * (op: 0 v_add_u32, 1 v_mul_lo_u32, 2 v_mul_hi_u32, 3 v_mad_u64_u32,
  * 4 v_mad_u32_u24, 5 v_lshl_add_u64, 6 v_cndmask_b32, 7 v_alignbit_b32,
  * 8 v_min_u32, 9 v_lshlrev_b64, 10 v_add_f64, 11 v_fma_f64, 12 v_mul_f64,
- * 13 v_max_f64) over independent chains on every CU.  The
+ * 13 v_max_f64, 14 v_cmp_eq_u32, 15 v_cmp_eq_f64, 16 v_cmp_eq_u64 (compares
+ * writing a lane mask to scalar registers), 17 v_xor_b32, 18 v_sub_u32,
+ * 19 v_add_f32) over independent chains on every CU.  The
  * measured integer peak the roofline is quoted against.  No reference
  * counterpart: the reference publishes no device peak. */
 int ctd_hip_valu_probe(ctd_hip_engine *eng, int op, uint32_t iters, double *lane_ops_per_s);

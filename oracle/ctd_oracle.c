@@ -386,9 +386,9 @@ typedef struct {
 } seqdef;
 
 /* value_mod_2 (number_sequence_helpers.hpp:14-24) per family:
- * mersenne.hpp:69-75, riesel_proth.hpp:199-206, riesel_proth_fixed_n.hpp:421-426,
- * smarandache.hpp:83-95, fibonacci.hpp:49-54, lucas.hpp:199-204,
- * tribonacci.hpp:372-376; the test sequence falls back to value_mod(n, 2). */
+ * mersenne.hpp:69-75, riesel_proth.hpp:74-81, riesel_proth_fixed_n.hpp:117-122,
+ * smarandache.hpp:83-95, fibonacci.hpp:49-54, lucas.hpp:40-45,
+ * tribonacci.hpp:104-108; the test sequence falls back to value_mod(n, 2). */
 uint64_t ctdo_value_mod_2(uint32_t seq, uint64_t param, uint64_t n)
 {
     switch (seq) {
@@ -409,9 +409,9 @@ uint64_t ctdo_value_mod_2(uint32_t seq, uint64_t param, uint64_t n)
 }
 
 /* next_value_mod_2 (number_sequence_helpers.hpp:26-37) per family:
- * mersenne.hpp:98-107, riesel_proth.hpp:208-213, riesel_proth_fixed_n.hpp:439-444,
- * smarandache.hpp:97-112, fibonacci.hpp:56-61, lucas.hpp:206-210,
- * tribonacci.hpp:378-383; the test sequence uses next_value_mod(r, n, 2). */
+ * mersenne.hpp:98-107, riesel_proth.hpp:83-88, riesel_proth_fixed_n.hpp:135-140,
+ * smarandache.hpp:97-112, fibonacci.hpp:56-61, lucas.hpp:47-51,
+ * tribonacci.hpp:110-115; the test sequence uses next_value_mod(r, n, 2). */
 uint64_t ctdo_next_value_mod_2(uint32_t seq, uint64_t param, uint64_t r, uint64_t n)
 {
     switch (seq) {
@@ -459,7 +459,7 @@ static fstate fib_state(uint64_t n, const bctx *c)
     return st;
 }
 
-/* (T(n), T(n+1), T(n+2)) mod d by the doubling of tribonacci.hpp:332-370 */
+/* (T(n), T(n+1), T(n+2)) mod d by the doubling of tribonacci.hpp:64-102 */
 static fstate trib_state(uint64_t n, const bctx *c)
 {
     uint64_t d = c->d;
@@ -497,13 +497,13 @@ static uint64_t seq_init(const seqdef *s, const bctx *c, fstate *st)
         if (n <= 1) return (n == 0 || d == 1) ? 0 : 1;
         r = modpow_b(2, n, c);
         return r ? r - 1 : d - 1;
-    case CTDO_PROTH: /* riesel_proth.hpp:215-226 */
+    case CTDO_PROTH: /* riesel_proth.hpp:90-101 */
         r = mulmod_b(modpow_b(2, n, c), s->param, c);
         return r == d - 1 ? 0 : r + 1;
     case CTDO_RIESEL:
         r = mulmod_b(modpow_b(2, n, c), s->param, c);
         return r ? r - 1 : d - 1;
-    case CTDO_PROTH_FIXED_N: /* riesel_proth_fixed_n.hpp:392-437 */
+    case CTDO_PROTH_FIXED_N: /* riesel_proth_fixed_n.hpp:88-94,124-133 */
     case CTDO_RIESEL_FIXED_N:
         st->a = modpow_b(2, s->param, c);
         r = mulmod_b(ctdo_mod(n, d, c->mu64), st->a, c);
@@ -514,16 +514,16 @@ static uint64_t seq_init(const seqdef *s, const bctx *c, fstate *st)
     case CTDO_FIBONACCI: /* fibonacci.hpp:64-96 */
         *st = fib_state(n, c);
         return st->b;
-    case CTDO_LUCAS: { /* lucas.hpp:213-231 */
+    case CTDO_LUCAS: { /* lucas.hpp:54-66 */
         fstate f = fib_state(n, c);
         st->a = submod(doublemod(f.b, d), f.a, d);
         st->b = addmod(f.b, doublemod(f.a, d), d);
         return st->b;
     }
-    case CTDO_TRIBONACCI: /* tribonacci.hpp:385-399 */
+    case CTDO_TRIBONACCI: /* tribonacci.hpp:117-131 */
         *st = trib_state(n, c);
         return st->a;
-    case CTDO_PEANO: /* tests/peano.hpp:228-236 */
+    case CTDO_PEANO: /* tests/peano.hpp:41-82 */
         return n % d;
     }
     return 0;
@@ -548,13 +548,13 @@ static uint64_t seq_next(const seqdef *s, uint64_t r, uint64_t n, const bctx *c,
             return t == d ? 0 : t;
         }
         return r - (d - r) + 1;
-    case CTDO_PROTH: /* riesel_proth.hpp:228-240 */
+    case CTDO_PROTH: /* riesel_proth.hpp:103-115 */
         r = (r < d - r) ? r << 1 : r - (d - r);
         return r ? r - 1 : d - 1;
     case CTDO_RIESEL:
         r = (r < d - r) ? r << 1 : r - (d - r);
         return r == d - 1 ? 0 : r + 1;
-    case CTDO_PROTH_FIXED_N: /* riesel_proth_fixed_n.hpp:446-455 */
+    case CTDO_PROTH_FIXED_N: /* riesel_proth_fixed_n.hpp:142-151 */
     case CTDO_RIESEL_FIXED_N:
         return (r >= d - st->a) ? r - (d - st->a) : r + st->a;
     case CTDO_SMARANDACHE: { /* smarandache.hpp:131-158 */
@@ -570,20 +570,20 @@ static uint64_t seq_next(const seqdef *s, uint64_t r, uint64_t n, const bctx *c,
         u128 rr = a - q * d;
         return (uint64_t)(rr >= d ? rr - d : rr);
     }
-    case CTDO_FIBONACCI: /* fibonacci.hpp:98-104, lucas.hpp:233-237 */
+    case CTDO_FIBONACCI: /* fibonacci.hpp:98-104, lucas.hpp:74-78 */
     case CTDO_LUCAS: {
         uint64_t res = addmod(r, st->a, d);
         st->a = r;
         return res;
     }
-    case CTDO_TRIBONACCI: { /* tribonacci.hpp:401-409 */
+    case CTDO_TRIBONACCI: { /* tribonacci.hpp:133-141 */
         uint64_t t3 = addmod(addmod(st->a, st->b, d), st->c, d);
         st->a = st->b;
         st->b = st->c;
         st->c = t3;
         return st->a;
     }
-    case CTDO_PEANO: /* tests/peano.hpp:272-281 */
+    case CTDO_PEANO: /* tests/peano.hpp:85-117 */
         r = r + 1;
         return r == d ? 0 : r;
     }

@@ -282,7 +282,7 @@ def test_smarandache_expression_reference_strings(built):
         assert run_cli("--num-seq", "smarandache", "--print-expression", n, check_rc=0).stdout.strip() == s
 
 
-# n lists of smarandache_gp_tests.cpp:153-215 (checked there against PARI)
+# n lists of smarandache_gp_tests.cpp:153-212 (checked there against PARI)
 GP_CASES = {
     10: [0, 1, 2, 3, 4, 5, 6, 9, 10, 11, 16, 29, 77, 100, 999, 1000, 1001],
     2: [0, 1, 2, 3, 4, 5, 7, 8, 9, 15, 16, 17, 31, 32, 33, 1647],

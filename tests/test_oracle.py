@@ -86,7 +86,7 @@ def test_mulmod_modpow_bruteforce():
         d = rng.getrandbits(rng.randint(2, 64)) | 1
         if d < 3:
             continue
-        a, b = rng.randrange(d), rng.getrandbits(64)  # b >= d occurs (mulmod(r, k) in riesel_proth.hpp:220)
+        a, b = rng.randrange(d), rng.getrandbits(64)  # b >= d occurs (mulmod(r, k) in riesel_proth.hpp:95)
         assert L.ctdo_mulmod(a, b, d) == (a * b) % d
         a2, b2 = rng.randrange(d), rng.randrange(d)
         assert L.ctdo_mulmod(a2, b2, d) == (a2 * b2) % d
@@ -187,7 +187,7 @@ def test_residues_bruteforce(seq, param):
 
 def test_fibonacci_lucas_n0_zero_quirk():
     # fibonacci.hpp:68-71 starts n = 0 from {previous, current} = {0, 0}
-    # (lucas.hpp:214-225 maps it to {0, 0} too): every later residue is 0
+    # (lucas.hpp:54-66 maps it to {0, 0} too): every later residue is 0
     for seq in (o.FIBONACCI, o.LUCAS):
         assert o.state(seq, 0, 0, 101) == [0, 0]
         assert o.residues(seq, 0, 0, 30, 101) == [0] * 30
