@@ -105,13 +105,17 @@ struct Tile {
     bool first;
 };
 
-double family_cost(uint32_t fam, bool fast)
+// VALU instructions per update of the family's loop (FP64 forms below 2^48,
+// integer forms above), for ordering tiles and balancing the streams.
+double family_cost(uint32_t fam, bool fast, bool fp64, uint64_t N)
 {
     if (!fast) return 50.0;
     switch (fam) {
-    case FAM_SMARANDACHE: return 13.0;
-    case FAM_TRIBONACCI: return 11.0;
-    default: return 7.0;
+    case FAM_SMARANDACHE: return fp64 ? 7.0 : 12.25;
+    case FAM_TRIBONACCI: return fp64 ? 6.0 : 11.25;
+    case FAM_FIBONACCI:
+    case FAM_LUCAS: return fp64 ? (N >= THETA_MIN_STEPS ? 2.1 : 5.0) : 6.0;
+    default: return fp64 ? 1.5 : 7.0;
     }
 }
 
@@ -298,7 +302,7 @@ int plan_jobs(ctd_hip_engine *E, const ctd_job *jobs, uint32_t njobs, uint32_t *
     for (auto &P : plans) {
         if (P.f0 >= P.f1) continue;
         const bool fast = Fast<0>::fits(P.f1 - 1);
-        P.ops = family_cost(P.J.fam, fast);
+        P.ops = family_cost(P.J.fam, fast, P.fp64 && P.f1 - 1 < FP_PMAX, P.N);
         if (!autotune) continue;
         const bool narrow = P.J.fam == FAM_SMARANDACHE && sm_narrow_ok(P.J.param, P.J.n1, P.f1 - 1);
         const uint64_t key = shape_key(P.J.fam, fast, narrow, P.N);
