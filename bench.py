@@ -19,6 +19,8 @@ device batch of 2^31 integers of the prime window:
 
 Calibration (--workload copy): one 256 MiB device-to-device copy, the
 reference kernel for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE to bytes.
+Probe (--workload probe): the issue rate of each VALU instruction the engine
+probes (ctd_hip_valu_probe).
 
 Serving workload (--workload serve): a Poisson stream of jobs drawn from the
 same generator, with 2^24-wide prime windows so that 10^4 jobs take about a
@@ -151,7 +153,7 @@ def same_buffers(c, f, rc, rf, mx):
 def run_mixed(args, torch, ctd, nat):
     from cutrialdive_amd import workloads
 
-    jobs = workloads.mixed_jobs(args.jobs)
+    jobs = workloads.mixed_jobs(args.jobs, window_bits=args.window_bits)
     if args.family:
         jobs = [j for j in jobs if j.kind == args.family]
     eng = ctd.Engine(0)
@@ -252,7 +254,7 @@ def run_mixed(args, torch, ctd, nat):
         "data": "synthetic (seeded job generator, seed %d)" % workloads.SEED,
         "config": {
             "workload": f"mixed batch of {len(jobs)} seeded jobs (7 families, 10^2..10^5 indices, "
-                        "2^30-wide prime windows in [2^38, 2^43)), batched form, one hipGraph launch",
+                        f"2^{args.window_bits}-wide prime windows in [2^38, 2^43)), batched form, one hipGraph launch",
             "jobs": len(jobs),
             "indices": int(sum(Ns)),
             "primes_timed": primes,
@@ -479,6 +481,19 @@ def run_copy(args, torch, ctd, nat):
     return 0
 
 
+def run_probe(args, torch, ctd, nat):
+    """Issue rate of each probed VALU instruction (lane-ops/s, best of three
+    8192-trip runs): the peaks the residue loops are measured against."""
+    eng = ctd.Engine(0)
+    out = {}
+    for op, name in enumerate(nat.PROBE_OPS):
+        out[name] = max(eng.valu_probe(op, 8192) for _ in range(3))
+        print(f"# {name}: {out[name] / 1e12:.2f} T lane-ops/s", file=sys.stderr, flush=True)
+    print(json.dumps({"workload": "probe", "unit": "lane-ops/s", "rates": out}), flush=True)
+    eng.close()
+    return 0
+
+
 def run_single(args, torch, ctd, nat):
     seq, param, n0, n1, F0, F1, batch, desc = SINGLE[args.workload]
     N = n1 - n0
@@ -597,7 +612,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (mixed: 1, single-job workloads: 30)")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="mixed", choices=["mixed", "serve", "copy"] + sorted(SINGLE))
+    ap.add_argument("--workload", default="mixed", choices=["mixed", "serve", "copy", "probe"] + sorted(SINGLE))
     ap.add_argument("--serve-jobs", type=int, default=10000, help="serve: jobs per offered load")
     ap.add_argument("--loads", default="0.5,0.7,0.9", help="serve: offered loads (fractions of capacity)")
     ap.add_argument("--capacity-jobs", type=int, default=2000, help="serve: jobs of the capacity measurement")
@@ -605,6 +620,7 @@ def main() -> int:
     ap.add_argument("--serve-cpu-jobs", type=int, default=12, help="serve: served jobs checked against the oracle")
     ap.add_argument("--jobs", type=int, default=256, help="mixed: jobs drawn from the seeded generator")
     ap.add_argument("--tile-bits", type=int, default=0, help="mixed: log2 of the tile length (0 = engine default)")
+    ap.add_argument("--window-bits", type=int, default=30, help="mixed: log2 of each job's prime window")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="mixed: issue the tiles eagerly (profilers)")
     ap.add_argument("--family", default=None, help="mixed: keep only the jobs of one family (per-family rates)")
@@ -629,6 +645,8 @@ def main() -> int:
         return run_serve(args, torch, ctd, nat)
     if args.workload == "copy":
         return run_copy(args, torch, ctd, nat)
+    if args.workload == "probe":
+        return run_probe(args, torch, ctd, nat)
     if args.workload == "mixed":
         if args.steps is None:
             args.steps = 1

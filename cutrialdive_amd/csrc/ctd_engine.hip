@@ -327,6 +327,10 @@ void build_base(ctd_hip_engine *E, uint64_t lim)
 {
     if (lim < 65536) lim = 65536;
     if (E->d_sp && lim <= E->base_lim) return;
+    // grow geometrically: consecutive batches of a window raise sqrt(f1) a
+    // little each time, and sieving numbers beyond sqrt(end) cost a segment
+    // nothing (the per-lane loop stops at the first one past it)
+    if (E->d_sp) lim = std::min<uint64_t>(std::max<uint64_t>(lim, 2 * E->base_lim), 1ull << 32);
     // level 0: odd numbers 3..255 sieve [0, 65537)
     uint32_t *l0 = nullptr;
     uint64_t *m0 = nullptr;
