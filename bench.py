@@ -238,7 +238,10 @@ def run_mixed(args, torch, ctd, nat):
     peak = probe_peak(eng)
     per_launch_s = bms / 1e3 / max(graphs, 1)
     achieved = upd / max(graphs, 1) * c_upd / per_launch_s / 1e12
-    traffic = 16.0 * primes / max(graphs, 1)  # sieve writes 8 B per prime, the residue kernel reads 8 B
+    sm = ops_table()["measured"]
+    # the sieve's measured write and the residue kernel's measured read per prime
+    traffic = (sm["sieve"]["hbm_write_bytes_per_prime"] + sm["sieve"]["hbm_read_bytes_per_prime"]
+               + sm["cfg2"]["hbm_read_bytes_per_prime"]) * primes / max(graphs, 1)
     result = {
         "metric": METRIC,
         "value": value,
@@ -274,7 +277,9 @@ def run_mixed(args, torch, ctd, nat):
             "unit": "T int32 VALU lane-ops/s",
             "frac": achieved / peak,
             "traffic": traffic,
-            "traffic_unit": "HBM bytes per graph launch (prime stream: sieve write + residue read, 8 B each)",
+            "traffic_unit": "HBM bytes per graph launch (prime stream: sieve write + residue read)",
+            "traffic_source": "per-prime bytes measured with rocprofv3 FETCH_SIZE x 2.0 / WRITE_SIZE on the "
+                              "same kernels, profiles/tf_fast_smarandache_cfg2.md",
             "kernel": "batch graph (sieve_kernel + tf_fast_kernel<family> per tile)",
             "ops_per_update": c_upd,
             "ops_per_update_source": c_src,
@@ -308,11 +313,20 @@ def run_mixed(args, torch, ctd, nat):
 
 
 def buffer_digest(c, f, mx):
-    """Order-free digest of a factors buffer (stored factors sorted per index)."""
+    """Order-free digest of a factors buffer: the counts, and the stored
+    factors of each index that has any, sorted (their order depends on the
+    atomics)."""
     import hashlib
 
-    h = hashlib.sha1(np.ascontiguousarray(c).tobytes())
-    h.update(np.sort(np.asarray(f).reshape(len(c), mx), axis=1).tobytes())
+    c = np.ascontiguousarray(np.asarray(c).view(np.uint32))
+    h = hashlib.sha1(c.tobytes())
+    nz = np.nonzero(c)[0]
+    if len(nz):
+        rows = np.asarray(f).view(np.uint64).reshape(len(c), mx)[nz]
+        k = np.minimum(c[nz], mx).astype(np.int64)
+        rows = np.where(np.arange(mx)[None, :] < k[:, None], rows, np.uint64(0))
+        h.update(nz.astype(np.int64).tobytes())
+        h.update(np.sort(rows, axis=1).tobytes())
     return h.hexdigest()
 
 
@@ -540,7 +554,13 @@ def run_single(args, torch, ctd, nat):
     peak = probe_peak(eng)
     avg_launch_s = tf_ms / 1e3 / max(launches, 1)
     achieved = updates / max(launches, 1) * c_upd / avg_launch_s / 1e12
-    traffic = 8.0 * primes / max(launches, 1)
+    tm = ops_table()["measured"].get(args.workload)
+    if tm:  # rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE of the residue kernel, per prime
+        traffic = (tm["hbm_read_bytes_per_prime"] + tm["hbm_write_bytes_per_prime"]) * primes / max(launches, 1)
+        traffic_src = "rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE (copy-calibrated), " + tm["profile"]
+    else:
+        traffic = 8.0 * primes / max(launches, 1)
+        traffic_src = "algorithmic: 8 B per prime read"
     result = {
         "metric": METRIC,
         "value": value,
@@ -564,7 +584,8 @@ def run_single(args, torch, ctd, nat):
             "unit": "T int32 VALU lane-ops/s",
             "frac": achieved / peak,
             "traffic": traffic,
-            "traffic_unit": "HBM bytes per launch (prime stream read; FETCH_SIZE x2 calibration)",
+            "traffic_unit": "HBM bytes per launch of the residue kernel",
+            "traffic_source": traffic_src,
             "kernel": f"tf_fast_kernel<{FAMILY_NAMES[seq]}>",
             "ops_per_update": c_upd,
             "ops_per_update_source": c_src,

@@ -804,10 +804,15 @@ int ctd_hip_server_poll(ctd_hip_server *srv, uint64_t ticket, int *state, double
 {
     if (!srv || !state) return CTD_E_INVALID;
     ctd_hip_server_impl *S = &srv->impl;
-    std::lock_guard<std::mutex> lk(S->mu);
+    std::unique_lock<std::mutex> lk(S->mu);
     auto it = S->tickets.find(ticket);
     if (it == S->tickets.end()) return fail(S->E, CTD_E_INVALID, "unknown ticket");
-    *state = it->second.state;
+    if (it->second.state == 3) {
+        // results staged: copy them into the caller's buffers now (another
+        // thread may be doing it; then report the job as still running)
+        deliver(S, it->second, lk);
+    }
+    *state = it->second.state == 3 ? 1 : it->second.state;
     if (latency_ms) *latency_ms = it->second.latency_ms;
     return CTD_OK;
 }
@@ -819,7 +824,12 @@ int ctd_hip_server_wait(ctd_hip_server *srv, uint64_t ticket, double *latency_ms
     std::unique_lock<std::mutex> lk(S->mu);
     auto it = S->tickets.find(ticket);
     if (it == S->tickets.end()) return fail(S->E, CTD_E_INVALID, "unknown ticket");
-    S->cv_done.wait(lk, [&] { return it->second.state == 2 || it->second.state < 0; });
+    for (;;) {
+        S->cv_done.wait(lk, [&] { return it->second.state >= 2 || it->second.state < 0; });
+        if (it->second.state != 3) break;
+        deliver(S, it->second, lk); // returns at once when another thread delivers it
+        if (it->second.state == 3) S->cv_done.wait(lk, [&] { return it->second.state != 3; });
+    }
     if (latency_ms) *latency_ms = it->second.latency_ms;
     if (it->second.state < 0) return fail(S->E, it->second.state, S->fatal_msg);
     return CTD_OK;
@@ -844,6 +854,12 @@ int ctd_hip_server_close(ctd_hip_server *srv)
         S->cv_work.notify_all();
     }
     if (S->th.joinable()) S->th.join();
+    {
+        // results nobody collected still reach the caller's buffers
+        std::unique_lock<std::mutex> lk(S->mu);
+        for (auto &kv : S->tickets)
+            if (kv.second.state == 3) deliver(S, kv.second, lk);
+    }
     int rc = S->fatal;
     if (rc == CTD_OK) {
         unsigned long long err = 0;

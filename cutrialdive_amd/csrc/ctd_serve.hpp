@@ -17,9 +17,15 @@
 //   retire  a tile is done when its event has completed; a job whose tiles are
 //           all done is copied back to its host buffers on the copy stream and
 //           completes when that copy has, which frees its device memory.
-// Latency is measured from submit to the moment the scheduler sees the copy
-// complete.  Residues, hits and factor buffers are those of the plain form:
-// the same kernels, the same tiles, only the interleaving differs.
+// Results travel device -> pinned staging buffer (asynchronous) -> the
+// caller's buffers.  The last copy is done by the thread that calls
+// ctd_hip_server_poll / ctd_hip_server_wait (or close), never by the
+// scheduler: a copy into pageable memory would block the scheduler thread
+// until the DMA finishes, and no tile could be issued meanwhile.  Latency is
+// measured from submit to the moment the results are back in host memory (the
+// staging copy has completed), whenever the caller collects them.
+// Residues, hits and factor buffers are those of the plain form: the same
+// kernels, the same tiles, only the interleaving differs.
 
 constexpr uint64_t SERVE_TILE_DEFAULT = 1ull << 26;
 constexpr uint64_t SERVE_BUDGET_DEFAULT = 8ull << 30;
@@ -61,6 +67,40 @@ struct DevicePool {
     }
 };
 
+// Pinned host staging buffers by power-of-two size class, kept for later
+// jobs (hipHostMalloc is slow and synchronizing).  Shared by the scheduler
+// and the delivering threads: used under the server mutex.
+struct PinnedPool {
+    std::multimap<uint64_t, void *> free_bufs;
+    std::vector<void *> all;
+    void *get(uint64_t bytes)
+    {
+        const uint64_t c = DevicePool::size_class(bytes);
+        auto it = free_bufs.find(c);
+        if (it != free_bufs.end()) {
+            void *p = it->second;
+            free_bufs.erase(it);
+            return p;
+        }
+        void *p = nullptr;
+        HIPCHK(hipHostMalloc(&p, c, hipHostMallocDefault));
+        all.push_back(p);
+        return p;
+    }
+    void put(void *p, uint64_t bytes) { free_bufs.emplace(DevicePool::size_class(bytes), p); }
+    void release()
+    {
+        for (void *p : all) (void)hipHostFree(p);
+        all.clear();
+        free_bufs.clear();
+    }
+};
+
+// Staging layout of one job: the two accumulators, then counts, then factors.
+inline uint64_t stage_counts_off() { return 2 * sizeof(unsigned long long); }
+inline uint64_t stage_factors_off(uint64_t N) { return (stage_counts_off() + N * sizeof(uint32_t) + 7) & ~7ull; }
+inline uint64_t stage_bytes(uint64_t N, uint32_t maxf) { return stage_factors_off(N) + N * (uint64_t)maxf * 8; }
+
 struct ServeJob {
     uint64_t ticket = 0;
     ctd_job job{};
@@ -69,7 +109,8 @@ struct ServeJob {
     uint32_t *d_counts = nullptr;
     uint64_t *d_factors = nullptr;
     unsigned long long *d_acc = nullptr; // odd primes tested, summed over the tiles
-    unsigned long long h_acc[2] = {0, 0};
+    unsigned char *stage = nullptr;      // pinned staging (stage_bytes)
+    bool delivering = false;
     uint64_t bytes = 0;
     JobPlan plan;
     std::deque<Tile> tiles;  // not yet issued
@@ -81,8 +122,9 @@ struct ServeJob {
 };
 
 struct Ticket {
-    int state = 0; // 0 queued, 1 running, 2 done, < 0 error code
+    int state = 0; // 0 queued, 1 running, 3 staged (results in pinned memory), 2 done, < 0 error code
     double latency_ms = 0;
+    ServeJob *staged = nullptr;
 };
 
 struct InFlight {
@@ -98,6 +140,7 @@ struct ctd_hip_server_impl {
     hipStream_t copy_st = nullptr;
     std::vector<hipEvent_t> event_pool;
     DevicePool pool;
+    PinnedPool pinned; // under mu
 
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -133,13 +176,45 @@ void finish_ticket(ctd_hip_server_impl *S, ServeJob *J, int state)
     Ticket &t = S->tickets[J->ticket];
     t.state = state;
     t.latency_ms = ms;
-    if (state == 2) {
-        S->stats.completed++;
-        S->stats.primes += J->h_acc[0];
-        S->stats.updates += J->h_acc[0] * (J->job.n1 > J->job.n0 ? J->job.n1 - J->job.n0 : 0);
-    } else {
-        S->stats.failed++;
-    }
+    if (state == 2) S->stats.completed++;
+    else S->stats.failed++;
+    S->cv_done.notify_all();
+}
+
+// The job's results are in its pinned staging buffer (scheduler thread):
+// count it and hand it to the delivering side.
+void stage_ticket(ctd_hip_server_impl *S, ServeJob *J)
+{
+    const unsigned long long *acc = (const unsigned long long *)J->stage;
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - J->t_submit).count();
+    std::lock_guard<std::mutex> lk(S->mu);
+    Ticket &t = S->tickets[J->ticket];
+    t.state = 3;
+    t.staged = J;
+    t.latency_ms = ms;
+    S->stats.completed++;
+    S->stats.primes += acc[0];
+    S->stats.updates += acc[0] * (J->job.n1 > J->job.n0 ? J->job.n1 - J->job.n0 : 0);
+    S->cv_done.notify_all();
+}
+
+// Copies a staged job's results into the caller's buffers (the calling
+// thread does the copy, outside the lock) and completes its ticket.  lk holds
+// S->mu on entry and on return.
+void deliver(ctd_hip_server_impl *S, Ticket &t, std::unique_lock<std::mutex> &lk)
+{
+    ServeJob *J = t.staged;
+    if (!J || J->delivering) return;
+    J->delivering = true;
+    lk.unlock();
+    const uint64_t N = J->job.n1 > J->job.n0 ? J->job.n1 - J->job.n0 : 0;
+    std::memcpy(J->h_counts, J->stage + stage_counts_off(), N * sizeof(uint32_t));
+    std::memcpy(J->h_factors, J->stage + stage_factors_off(N), N * (uint64_t)J->job.max_factors * 8);
+    lk.lock();
+    S->pinned.put(J->stage, stage_bytes(N, J->job.max_factors));
+    t.staged = nullptr;
+    t.state = 2;
+    delete J;
     S->cv_done.notify_all();
 }
 
@@ -165,6 +240,10 @@ bool admit(ctd_hip_server_impl *S, ServeJob *J)
     J->d_counts = (uint32_t *)S->pool.get(N * sizeof(uint32_t));
     J->d_factors = (uint64_t *)S->pool.get(N * (uint64_t)q.max_factors * sizeof(uint64_t));
     J->d_acc = (unsigned long long *)S->pool.get(2 * sizeof(unsigned long long));
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        J->stage = (unsigned char *)S->pinned.get(stage_bytes(N, q.max_factors));
+    }
     S->used_bytes += J->bytes;
     hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, S->copy_st, J->d_acc, 2u);
     if (S->opts.flags & CTD_SERVE_FRESH_BUFFERS) {
@@ -175,8 +254,13 @@ bool admit(ctd_hip_server_impl *S, ServeJob *J)
         hipLaunchKernelGGL(zero_words_kernel, dim3((uint32_t)std::min<uint64_t>(div_up(wf, 1024), 4096)), dim3(256),
                            0, S->copy_st, (uint32_t *)J->d_factors, wf);
     } else {
-        HIPCHK(hipMemcpyAsync(J->d_counts, J->h_counts, N * sizeof(uint32_t), hipMemcpyHostToDevice, S->copy_st));
-        HIPCHK(hipMemcpyAsync(J->d_factors, J->h_factors, N * (uint64_t)q.max_factors * sizeof(uint64_t),
+        // the caller's buffers go through the pinned staging buffer, so the
+        // copies to the device are asynchronous
+        std::memcpy(J->stage + stage_counts_off(), J->h_counts, N * sizeof(uint32_t));
+        std::memcpy(J->stage + stage_factors_off(N), J->h_factors, N * (uint64_t)q.max_factors * 8);
+        HIPCHK(hipMemcpyAsync(J->d_counts, J->stage + stage_counts_off(), N * sizeof(uint32_t),
+                              hipMemcpyHostToDevice, S->copy_st));
+        HIPCHK(hipMemcpyAsync(J->d_factors, J->stage + stage_factors_off(N), N * (uint64_t)q.max_factors * 8,
                               hipMemcpyHostToDevice, S->copy_st));
     }
     std::vector<JobPlan> plans;
@@ -283,11 +367,13 @@ bool schedule_once(ctd_hip_server_impl *S)
         const uint64_t N = J->job.n1 - J->job.n0;
         if (!J->copying) {
             // every tile event has completed, so the device buffers are final
-            HIPCHK(hipMemcpyAsync(J->h_counts, J->d_counts, N * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                                  S->copy_st));
-            HIPCHK(hipMemcpyAsync(J->h_factors, J->d_factors, N * (uint64_t)J->job.max_factors * sizeof(uint64_t),
+            HIPCHK(hipMemcpyAsync(J->stage + stage_counts_off(), J->d_counts, N * sizeof(uint32_t),
                                   hipMemcpyDeviceToHost, S->copy_st));
-            HIPCHK(hipMemcpyAsync(J->h_acc, J->d_acc, sizeof J->h_acc, hipMemcpyDeviceToHost, S->copy_st));
+            HIPCHK(hipMemcpyAsync(J->stage + stage_factors_off(N), J->d_factors,
+                                  N * (uint64_t)J->job.max_factors * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                                  S->copy_st));
+            HIPCHK(hipMemcpyAsync(J->stage, J->d_acc, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                  S->copy_st));
             J->done = pool_event(S);
             HIPCHK(hipEventRecord(J->done, S->copy_st));
             J->copying = true;
@@ -305,9 +391,8 @@ bool schedule_once(ctd_hip_server_impl *S)
         S->used_bytes -= J->bytes;
         S->event_pool.push_back(J->ready);
         S->event_pool.push_back(J->done);
-        finish_ticket(S, J, 2);
-        delete J;
         it = S->active.erase(it);
+        stage_ticket(S, J); // J now belongs to the delivering side
         progress = true;
     }
     (void)E;
@@ -322,6 +407,7 @@ void fail_all(ctd_hip_server_impl *S, int code, const std::string &msg)
     for (ServeJob *J : S->active) {
         S->tickets[J->ticket].state = code;
         S->stats.failed++;
+        if (J->stage) S->pinned.put(J->stage, stage_bytes(J->job.n1 - J->job.n0, J->job.max_factors));
         delete J;
     }
     S->active.clear();
@@ -374,4 +460,9 @@ void release_server(ctd_hip_server_impl *S)
     }
     for (auto e : S->event_pool) (void)hipEventDestroy(e);
     S->pool.release();
+    for (auto &kv : S->tickets) { // staged results nobody collected
+        if (kv.second.staged) delete kv.second.staged;
+        kv.second.staged = nullptr;
+    }
+    S->pinned.release();
 }

@@ -220,13 +220,17 @@ typedef struct ctd_server_stats {
 } ctd_server_stats;
 
 int ctd_hip_server_open(ctd_hip_engine *eng, const ctd_server_opts *opts, ctd_hip_server **out);
-/* counts/factors: the job's host factors buffer (read at admission, written at
- * completion); it must stay valid until the job is done.  Returns CTD_E_BUSY
- * when queue_limit jobs are already waiting. */
+/* counts/factors: the job's host factors buffer (read at admission, written
+ * when the job is collected); it must stay valid until the job is done.  The
+ * device results go to a pinned staging buffer of the server; the thread that
+ * collects the job (poll reporting done, wait, or close) copies them into
+ * counts/factors.  Returns CTD_E_BUSY when queue_limit jobs are already
+ * waiting. */
 int ctd_hip_server_submit(ctd_hip_server *srv, const ctd_job *job, uint32_t *counts,
                           uint64_t *factors, uint64_t *ticket);
-/* *state: 0 queued, 1 admitted, 2 done, negative: the CTD_E_* code of a failure.
- * *latency_ms: submit to completion, once done. */
+/* *state: 0 queued, 1 admitted, 2 done (the results are in the job's
+ * buffers), negative: the CTD_E_* code of a failure.  *latency_ms: submit to
+ * the moment the results were back in host memory, once done. */
 int ctd_hip_server_poll(ctd_hip_server *srv, uint64_t ticket, int *state, double *latency_ms);
 /* Blocks until the job is done (or failed). */
 int ctd_hip_server_wait(ctd_hip_server *srv, uint64_t ticket, double *latency_ms);

@@ -138,3 +138,23 @@ def test_fresh_buffers_ignore_host_contents(engine):
     rc, rf = oracle.trial_factor_raw(second.seq_id, second.param, second.n0, second.n1, second.f0, second.f1, 16)
     assert np.array_equal(c, rc) and np.array_equal(np.sort(f.reshape(398, 16), axis=1),
                                                     np.sort(rf.reshape(398, 16), axis=1))
+
+
+def test_poll_delivers_and_close_collects(engine):
+    """Results reach the caller's buffers when poll reports a job done, and
+    close() delivers the results of jobs nobody collected."""
+    jobs = small_jobs(8, seed=3, window_bits=18, nmax=300)
+    bufs = [(np.zeros(s.n1 - s.n0, np.uint32), np.zeros((s.n1 - s.n0) * 64, np.uint64)) for s in jobs]
+    with ctd.Server(engine, fresh_buffers=True) as srv:
+        tickets = [srv.submit(job_of(s), c, f) for s, (c, f) in zip(jobs, bufs)]
+        deadline = time.time() + 60
+        while srv.poll(tickets[0])[0] != 2:
+            assert time.time() < deadline
+            time.sleep(0.001)
+        c, f = bufs[0]
+        s = jobs[0]
+        rc, rf = oracle.trial_factor_raw(s.seq_id, s.param, s.n0, s.n1, s.f0, s.f1, 64)
+        assert_same(c, f, rc, rf, 64, "poll")
+    for s, (c, f) in zip(jobs[1:], bufs[1:]):
+        rc, rf = oracle.trial_factor_raw(s.seq_id, s.param, s.n0, s.n1, s.f0, s.f1, 64)
+        assert_same(c, f, rc, rf, 64, "close")
