@@ -112,7 +112,7 @@ double family_cost(uint32_t fam, bool fast, bool fp64, uint64_t N)
     if (!fast) return 50.0;
     switch (fam) {
     case FAM_SMARANDACHE: return fp64 ? 7.0 : 12.25;
-    case FAM_TRIBONACCI: return fp64 ? 6.0 : 11.25;
+    case FAM_TRIBONACCI: return fp64 ? 5.5 : 11.25;
     case FAM_FIBONACCI:
     case FAM_LUCAS: return fp64 ? (N >= THETA_MIN_STEPS ? 2.1 : 5.0) : 6.0;
     default: return fp64 ? 1.5 : 7.0;

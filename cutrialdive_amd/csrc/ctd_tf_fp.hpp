@@ -38,7 +38,7 @@
 //                                  (integer W64 form: 7)
 //   fixed n  x = x + c        1.25 (6)
 //   fibonacci / lucas         5   (6)
-//   tribonacci                6   (11.25)
+//   tribonacci                5.5 (11.25)
 //   smarandache r m + n + 1   7   (12.25), for p > 8 n1
 //
 // and the per-prime set-up (powers, doubling formulas, Smarandache jump) runs
@@ -396,13 +396,21 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
         double a, b, c;
         trib_triple_fp(F, ns, a, b, c);
         record_hits(J, idx, p, a == 0.0);
+        // two steps share one sum: with s = T(n+1) + T(n+2),
+        // T(n+3) = T(n) + s and T(n+4) = s + T(n+3) (11 instructions per 2)
         CTD_UNROLLED_LOOP(
             {
-                a = F.red(a + b + c); h0 = b == 0.0;
-                b = F.red(b + c + a); h1 = c == 0.0;
-                c = F.red(c + a + b); h2 = a == 0.0;
-                const double t = F.red(a + b + c);
-                a = b; b = c; c = t; h3 = a == 0.0;
+                double s = b + c;
+                const double d = F.red(a + s);
+                const double e = F.red(s + d);
+                h0 = b == 0.0; h1 = c == 0.0;
+                s = d + e;
+                a = F.red(c + s);
+                b = F.red(s + a);
+                c = e;
+                h2 = d == 0.0; h3 = e == 0.0;
+                // (a, b, c) = (T(n+5), T(n+6), T(n+4)): rotate to (T(n+4), T(n+5), T(n+6))
+                const double t = c; c = b; b = a; a = t;
             },
             { const double t = F.red(a + b + c); a = b; b = c; c = t; }, a == 0.0)
         return F.canon(a, p);
