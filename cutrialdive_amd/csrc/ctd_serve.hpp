@@ -305,15 +305,23 @@ bool schedule_once(ctd_hip_server_impl *S)
 {
     ctd_hip_engine *E = S->E;
     bool progress = false;
-    // admit
+    // admit, while the admitted jobs hold fewer unissued tiles than twice what
+    // the streams keep in flight: enough to keep every stream fed, and jobs
+    // that could not start yet hold no device memory
+    uint64_t ready_tiles = 0;
+    for (ServeJob *a : S->active) ready_tiles += a->tiles.size();
+    const uint64_t want = 2ull * S->slots.size() * SERVE_DEPTH;
     for (;;) {
+        if (ready_tiles >= want) break;
         ServeJob *J = nullptr;
         {
             std::lock_guard<std::mutex> lk(S->mu);
             if (S->pending.empty()) break;
             J = S->pending.front();
         }
-        if (!admit(S, J)) break;
+        const size_t before = S->active.size();
+        if (!admit(S, J)) break; // an empty job is finished (and freed) by admit
+        if (S->active.size() > before) ready_tiles += S->active.back()->tiles.size();
         std::lock_guard<std::mutex> lk(S->mu);
         S->pending.pop_front();
         progress = true;
