@@ -402,10 +402,21 @@ def run_serve(args, torch, ctd, nat):
     _, _, cap_s, cap_st = serve(cjobs[:ncap], None)
     capacity = ncap / cap_s
     print(f"# capacity {capacity:.1f} jobs/s ({cap_st.updates / cap_s:.3e} updates/s)", file=sys.stderr, flush=True)
-    # the same jobs in the batched form (one graph), for comparison
+    # the same jobs in the batched form (one graph, factors buffers resident
+    # in HBM), for comparison; the serving form also returns every job's
+    # buffers to host memory
+    Ns = [c.n1 - c.n0 for c in cjobs[:ncap]]
+    offs = np.concatenate([[0], np.cumsum(Ns)])
+    d_counts = torch.zeros(int(offs[-1]), dtype=torch.int32, device="cuda")
+    d_factors = torch.zeros(int(offs[-1]) * mx, dtype=torch.int64, device="cuda")
+    cptr = [d_counts.data_ptr() + 4 * int(offs[k]) for k in range(ncap)]
+    fptr = [d_factors.data_ptr() + 8 * mx * int(offs[k]) for k in range(ncap)]
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    _, bst = eng.run_batch(cjobs[:ncap], tile_length=1 << 24)
+    bst = eng.run_batch_raw(cjobs[:ncap], cptr, fptr, 0, 1 << 24)
+    torch.cuda.synchronize()
     batch_s = time.perf_counter() - t0
+    del d_counts, d_factors
     print(f"# batched form on the same jobs: {ncap / batch_s:.1f} jobs/s ({bst.updates / batch_s:.3e} updates/s)",
           file=sys.stderr, flush=True)
 
