@@ -16,6 +16,9 @@ device batch of 2^31 integers of the prime window:
   cfg1   Mersenne, n in [2, 20000), primes < 2^30
   cfg3a  Sm16, n in [1, 2000), primes in [2^40, 2^40 + 2^36)
   cfg3b  Proth k=3, n in [1, 10^5), primes in [2^40, 2^40 + 2^32)
+  readme35 / readme36  the reference README's benchmark job, Proth
+         k = 2718281828459, n in [5e7, 5e7 + 5e5), all primes below 2^35 / 2^36;
+         vs_baseline is the ratio to the reference's published rate for it
 
 Calibration (--workload copy): one 256 MiB device-to-device copy, the
 reference kernel for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE to bytes.
@@ -55,7 +58,15 @@ SINGLE = {
               "smarandache,16 n in [1,2000), primes in [2^40, 2^40+2^36) (sub-window of [2^40,2^42))"),
     "cfg3b": (1, 3, 1, 100000, 1 << 40, (1 << 40) + (1 << 32), 1 << 28,
               "proth,3 n in [1,100000), primes in [2^40, 2^40+2^32) (sub-window of [2^40,2^42))"),
+    # the reference's own benchmark job (README.md:127-141), whole TF ranges
+    "readme35": (1, 2718281828459, 50000000, 50500000, 0, 1 << 35, 1 << 31,
+                 "proth,2718281828459 n in [5e7, 5e7+5e5), primes < 2^35 (the reference README's benchmark job)"),
+    "readme36": (1, 2718281828459, 50000000, 50500000, 0, 1 << 36, 1 << 31,
+                 "proth,2718281828459 n in [5e7, 5e7+5e5), primes < 2^36 (the reference README's benchmark job)"),
 }
+# Published rates of the reference's GPU engine for the same jobs
+# ((candidate x prime)/s = updates/s, GeForce RTX 5090, README.md:138-139)
+PUBLISHED = {"readme35": 1.39e12, "readme36": 1.74e12}
 FAMILY_NAMES = {0: "mersenne", 1: "proth", 2: "riesel", 3: "smarandache", 4: "fibonacci", 5: "lucas",
                 6: "proth_fixed_n", 7: "riesel_fixed_n", 8: "tribonacci"}
 METRIC = "prime×index residue updates/s, batched vs plain on same GPU, bit-exact factors"
@@ -539,7 +550,7 @@ def run_single(args, torch, ctd, nat):
     N = n1 - n0
     mx = args.max_factors
     nbatches = (F1 - F0 + batch - 1) // batch
-    steps = args.steps if args.steps is not None else 30
+    steps = args.steps if args.steps is not None else min(30, nbatches)
     eng = ctd.Engine(0)
     name, mem, cu = eng.device_info()
     print(f"# device {name}, {cu} CUs, {mem / 2**30:.0f} GiB; workload {args.workload}: {desc}", file=sys.stderr,
@@ -597,7 +608,10 @@ def run_single(args, torch, ctd, nat):
         "ms_per_step": dt * 1e3 / steps,
         "higher_is_better": True,
         "scaling": "weak",
-        "vs_baseline": None,
+        "vs_baseline": (value / PUBLISHED[args.workload]) if args.workload in PUBLISHED and steps == nbatches else None,
+        "vs_baseline_note": ("published rate of the reference's GPU engine on the same job and TF range: "
+                             f"{PUBLISHED[args.workload]:.3g} updates/s, GeForce RTX 5090 (README.md:138-139)")
+                            if args.workload in PUBLISHED else None,
         "dtype": "f64",
         "dtype_note": "residues are exact integers below 2^53 carried in FP64 registers (nothing is rounded); "
                       "u64 integer forms for primes at or above 2^48",
@@ -706,7 +720,8 @@ def main() -> int:
             args.cpu_sample_bits = 16
         return run_mixed(args, torch, ctd, nat)
     if args.cpu_sample_bits is None:
-        args.cpu_sample_bits = 29
+        # about 10-30 s of oracle work on 16 threads
+        args.cpu_sample_bits = {"cfg1": 26, "cfg2": 29, "cfg3a": 24, "cfg3b": 20}.get(args.workload, 17)
     return run_single(args, torch, ctd, nat)
 
 
