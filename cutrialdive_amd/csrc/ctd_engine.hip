@@ -807,13 +807,14 @@ int ctd_hip_server_poll(ctd_hip_server *srv, uint64_t ticket, int *state, double
     std::unique_lock<std::mutex> lk(S->mu);
     auto it = S->tickets.find(ticket);
     if (it == S->tickets.end()) return fail(S->E, CTD_E_INVALID, "unknown ticket");
-    if (it->second.state == 3) {
+    Ticket &t = it->second; // stays valid while deliver() drops the lock
+    if (t.state == 3) {
         // results staged: copy them into the caller's buffers now (another
         // thread may be doing it; then report the job as still running)
-        deliver(S, it->second, lk);
+        deliver(S, t, lk);
     }
-    *state = it->second.state == 3 ? 1 : it->second.state;
-    if (latency_ms) *latency_ms = it->second.latency_ms;
+    *state = t.state == 3 ? 1 : t.state;
+    if (latency_ms) *latency_ms = t.latency_ms;
     return CTD_OK;
 }
 
@@ -824,14 +825,17 @@ int ctd_hip_server_wait(ctd_hip_server *srv, uint64_t ticket, double *latency_ms
     std::unique_lock<std::mutex> lk(S->mu);
     auto it = S->tickets.find(ticket);
     if (it == S->tickets.end()) return fail(S->E, CTD_E_INVALID, "unknown ticket");
+    // a reference, not the iterator: submits on other threads may rehash the
+    // map while this one waits, which keeps references valid but not iterators
+    Ticket &t = it->second;
     for (;;) {
-        S->cv_done.wait(lk, [&] { return it->second.state >= 2 || it->second.state < 0; });
-        if (it->second.state != 3) break;
-        deliver(S, it->second, lk); // returns at once when another thread delivers it
-        if (it->second.state == 3) S->cv_done.wait(lk, [&] { return it->second.state != 3; });
+        S->cv_done.wait(lk, [&] { return t.state >= 2 || t.state < 0; });
+        if (t.state != 3) break;
+        deliver(S, t, lk); // returns at once when another thread delivers it
+        if (t.state == 3) S->cv_done.wait(lk, [&] { return t.state != 3; });
     }
-    if (latency_ms) *latency_ms = it->second.latency_ms;
-    if (it->second.state < 0) return fail(S->E, it->second.state, S->fatal_msg);
+    if (latency_ms) *latency_ms = t.latency_ms;
+    if (t.state < 0) return fail(S->E, t.state, S->fatal_msg);
     return CTD_OK;
 }
 
@@ -857,8 +861,10 @@ int ctd_hip_server_close(ctd_hip_server *srv)
     {
         // results nobody collected still reach the caller's buffers
         std::unique_lock<std::mutex> lk(S->mu);
+        std::vector<Ticket *> staged;
         for (auto &kv : S->tickets)
-            if (kv.second.state == 3) deliver(S, kv.second, lk);
+            if (kv.second.state == 3) staged.push_back(&kv.second);
+        for (Ticket *t : staged) deliver(S, *t, lk);
     }
     int rc = S->fatal;
     if (rc == CTD_OK) {
