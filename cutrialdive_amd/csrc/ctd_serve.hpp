@@ -7,10 +7,11 @@
 //
 // Scheduler loop (one host thread, no device-side waits):
 //   admit   pending jobs while their factors buffers fit in the device-memory
-//           budget (the first job is always admitted): allocate the device
-//           buffers, copy the host buffers in, run the mod-2 kernel, and cut
-//           the prime window into tiles;
-//   issue   each stream keeps at most two tiles in flight; a free place goes
+//           budget (the first job is always admitted) and the streams need
+//           work: allocate the device buffers, copy the host buffers in, run
+//           the mod-2 kernel, and cut the prime window into tiles of about
+//           2^33 updates (at most tile_length integers);
+//   issue   each stream keeps at most SERVE_DEPTH tiles in flight; a free place goes
 //           to the oldest admitted job that still has tiles, so one job can
 //           spread over every stream and jobs that arrive later join as soon
 //           as the earlier ones have issued all their tiles;
@@ -28,6 +29,7 @@
 // kernels, the same tiles, only the interleaving differs.
 
 constexpr uint64_t SERVE_TILE_DEFAULT = 1ull << 26;
+constexpr double SERVE_TILE_UPDATES = 8589934592.0; // 2^33 updates: ~1 ms of the whole device
 constexpr uint64_t SERVE_BUDGET_DEFAULT = 8ull << 30;
 constexpr uint32_t SERVE_QUEUE_DEFAULT = 65536;
 constexpr uint32_t SERVE_DEPTH = 4; // tiles in flight per stream
@@ -281,7 +283,13 @@ bool admit(ctd_hip_server_impl *S, ServeJob *J)
     }
     J->ready = pool_event(S);
     HIPCHK(hipEventRecord(J->ready, S->copy_st));
-    const uint64_t tl = S->opts.tile_length;
+    // tiles of about SERVE_TILE_UPDATES updates (at most tile_length
+    // integers, at least one segment): a job with many indices is cut finer,
+    // so its tiles spread over every stream instead of holding one
+    const double primes_per_int = 1.0 / std::log((double)std::max<uint64_t>(q.f1, 16));
+    const double want = SERVE_TILE_UPDATES / ((double)N * primes_per_int); // integers
+    const uint64_t segs = std::max<uint64_t>(1, (uint64_t)(want / (double)SEG_SPAN));
+    const uint64_t tl = std::min<uint64_t>(S->opts.tile_length, segs * SEG_SPAN);
     if (J->plan.f0 < J->plan.f1) {
         const uint64_t lo0 = J->plan.f0 & ~1ull;
         for (uint64_t blo = lo0; blo < J->plan.f1; blo += tl) {
