@@ -16,8 +16,9 @@ device batch of 2^31 integers of the prime window:
   cfg1   Mersenne, n in [2, 20000), primes < 2^30
   cfg3a  Sm16, n in [1, 2000), primes in [2^40, 2^40 + 2^36)
   cfg3b  Proth k=3, n in [1, 10^5), primes in [2^40, 2^40 + 2^32)
-  readme35 / readme36  the reference README's benchmark job, Proth
-         k = 2718281828459, n in [5e7, 5e7 + 5e5), all primes below 2^35 / 2^36;
+  readme35 / readme36 / readme37 / readme40  the reference README's benchmark
+         job, Proth k = 2718281828459, n in [5e7, 5e7 + 5e5), all primes below
+         2^35 / 2^36 / 2^37 / 2^40 (about 0.5 / 1 / 2 / 15 minutes);
          vs_baseline is the ratio to the reference's published rate for it
 
 Calibration (--workload copy): one 256 MiB device-to-device copy, the
@@ -63,14 +64,19 @@ SINGLE = {
                  "proth,2718281828459 n in [5e7, 5e7+5e5), primes < 2^35 (the reference README's benchmark job)"),
     "readme36": (1, 2718281828459, 50000000, 50500000, 0, 1 << 36, 1 << 31,
                  "proth,2718281828459 n in [5e7, 5e7+5e5), primes < 2^36 (the reference README's benchmark job)"),
+    "readme37": (1, 2718281828459, 50000000, 50500000, 0, 1 << 37, 1 << 31,
+                 "proth,2718281828459 n in [5e7, 5e7+5e5), primes < 2^37 (the reference README's benchmark job)"),
+    "readme40": (1, 2718281828459, 50000000, 50500000, 0, 1 << 40, 1 << 31,
+                 "proth,2718281828459 n in [5e7, 5e7+5e5), primes < 2^40 (the reference README's benchmark job)"),
 }
 # Published rates of the reference's GPU engine for the same jobs
-# ((candidate x prime)/s = updates/s, GeForce RTX 5090, README.md:138-139)
-PUBLISHED = {"readme35": 1.39e12, "readme36": 1.74e12}
+# ((candidate x prime)/s = updates/s, GeForce RTX 5090, README.md:138-141)
+PUBLISHED = {"readme35": 1.39e12, "readme36": 1.74e12, "readme37": 1.98e12, "readme40": 2.28e12}
 FAMILY_NAMES = {0: "mersenne", 1: "proth", 2: "riesel", 3: "smarandache", 4: "fibonacci", 5: "lucas",
                 6: "proth_fixed_n", 7: "riesel_fixed_n", 8: "tribonacci"}
 METRIC = "prime×index residue updates/s, batched vs plain on same GPU, bit-exact factors"
 HBM_PEAK_GBPS = 8000.0
+SM_SHORT_LIMIT = 2.0 ** 53 - 2.0 ** 30  # p (0.75 m + 1) below it: the short Smarandache step
 
 
 def ops_table():
@@ -96,24 +102,48 @@ def loop_ops(t, fam: str, pmax: int, pmin: int, n_items: int = 0) -> float:
     return float(e["w32"] if pmax < (1 << 31) else e["w64"])
 
 
-def window_ops(t, fam: str, lo: int, hi: int, n_items: int = 0) -> float:
-    """Prime-count-weighted loop ops per update over the window [lo, hi)."""
-    cuts = sorted({lo, hi} | {1 << b for b in (31, 35, 37, 45, 48) if lo < (1 << b) < hi})
+def sm_multipliers(base: int, n0: int, n1: int):
+    """[(m, steps)]: the Smarandache steps n -> n+1 over [n0, n1) grouped by
+    their multiplier m = base^digits(n+1)."""
+    out = []
+    m, lo = base, 1  # numbers in [lo, m) have the digits of m
+    while lo < n1:
+        a, b = max(lo, n0 + 1), min(m, n1)
+        if a < b:
+            out.append((m, b - a))
+        lo, m = m, m * base
+    return out
+
+
+def window_ops(t, fam: str, lo: int, hi: int, n_items: int = 0, sm=None) -> float:
+    """Prime-count-weighted loop ops per update over the window [lo, hi).
+    sm = (base, n0, n1) for a Smarandache job: its FP64 form takes the short
+    step on the primes below 2^53 / (0.75 m + 1) of each multiplier m."""
+    cuts = {lo, hi} | {1 << b for b in (31, 35, 37, 45, 48) if lo < (1 << b) < hi}
+    mults = sm_multipliers(*sm) if (fam == "smarandache" and sm) else []
+    lims = {m: int(SM_SHORT_LIMIT / (0.75 * m + 1)) for m, _ in mults}
+    cuts |= {x for x in lims.values() if lo < x < hi}
+    cuts = sorted(cuts)
     num = den = 0.0
     for a, b in zip(cuts, cuts[1:]):
         w = (b - a) / math.log(max(b, 16))
-        num += w * loop_ops(t, fam, b - 1, a, n_items)
+        ops = loop_ops(t, fam, b - 1, a, n_items)
+        if mults and b - 1 < (1 << 48):
+            short = float(t["loop"][fam]["fp64_short"])
+            steps = sum(c for _, c in mults)
+            ops = sum(c * (short if b <= lims[m] else ops) for m, c in mults) / steps
+        num += w * ops
         den += w
     return num / den
 
 
-def ops_per_update(workload: str, seq: int, lo: int, hi: int, n_items: int):
+def ops_per_update(workload: str, seq: int, lo: int, hi: int, n_items: int, sm=None):
     """(VALU lane-ops per update, source) for a single-job workload."""
     t = ops_table()
     m = t["measured"].get(workload)
     if m:
         return float(m["valu_lane_ops_per_update"]), "rocprofv3 " + m["profile"]
-    return window_ops(t, FAMILY_NAMES[seq], max(lo, 3), hi, n_items), "loop ISA count, prime-weighted"
+    return window_ops(t, FAMILY_NAMES[seq], max(lo, 3), hi, n_items, sm), "loop ISA count, prime-weighted"
 
 
 def mixed_ops(jobs, full_batch=True):
@@ -125,7 +155,7 @@ def mixed_ops(jobs, full_batch=True):
     num = den = 0.0
     for j in jobs:
         u = (j.f1 - j.f0) / math.log(j.f1) * (j.n1 - j.n0)
-        num += u * window_ops(t, FAMILY_NAMES[j.seq_id], j.f0, j.f1, j.n1 - j.n0)
+        num += u * window_ops(t, FAMILY_NAMES[j.seq_id], j.f0, j.f1, j.n1 - j.n0, (j.param, j.n0, j.n1))
         den += u
     return num / den, "loop ISA count, update-weighted"
 
@@ -588,7 +618,7 @@ def run_single(args, torch, ctd, nat):
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     value = updates / dt
-    c_upd, c_src = ops_per_update(args.workload, seq, F0, F1, N)
+    c_upd, c_src = ops_per_update(args.workload, seq, F0, F1, N, (param, n0, n1))
     peak, peak_add = probe_peak(eng)
     avg_launch_s = tf_ms / 1e3 / max(launches, 1)
     achieved = updates / max(launches, 1) * c_upd / avg_launch_s / 1e12
@@ -611,7 +641,7 @@ def run_single(args, torch, ctd, nat):
         "scaling": "weak",
         "vs_baseline": (value / PUBLISHED[args.workload]) if args.workload in PUBLISHED and steps == nbatches else None,
         "vs_baseline_note": ("published rate of the reference's GPU engine on the same job and TF range: "
-                             f"{PUBLISHED[args.workload]:.3g} updates/s, GeForce RTX 5090 (README.md:138-139)")
+                             f"{PUBLISHED[args.workload]:.3g} updates/s, GeForce RTX 5090 (README.md:138-141)")
                             if args.workload in PUBLISHED else None,
         "dtype": "f64",
         "dtype_note": "residues are exact integers below 2^53 carried in FP64 registers (nothing is rounded); "

@@ -25,6 +25,7 @@ CTD_E_BUSY = -5
 
 CTD_RUN_GENERIC_ONLY = 1
 CTD_RUN_NO_FP64 = 2
+CTD_RUN_SM_FULL_STEP = 4
 CTD_BATCH_HOST_BUFFERS = 1
 CTD_BATCH_NO_GRAPH = 2
 CTD_BATCH_NO_AUTOTUNE = 4

@@ -543,6 +543,7 @@ int run_job(ctd_hip_engine *E, const ctd_job *job, const ctd_run_opts *opts, con
                 J.nchunks = (uint32_t)div_up(N, chunk);
                 J.flags = (J.fam == FAM_SMARANDACHE && sm_narrow_ok(J.param, J.n1, bhi - 1)) ? JOB_SM_NARROW : 0u;
                 if (bhi - 1 < FP_PMAX && !(opts && (opts->flags & CTD_RUN_NO_FP64))) J.flags |= JOB_FP64;
+                if (opts && (opts->flags & CTD_RUN_SM_FULL_STEP)) J.flags |= JOB_SM_FULL_STEP;
                 uint32_t block = (opts && opts->block_size > 0) ? (uint32_t)opts->block_size : 256u;
                 block = std::max<uint32_t>(64, std::min<uint32_t>(block, 256)) & ~63u;
                 uint64_t items = ((np + 63) & ~63ull) * nch;
@@ -758,6 +759,7 @@ int ctd_hip_server_open(ctd_hip_engine *E, const ctd_server_opts *opts, ctd_hip_
         HIPCHK(hipStreamCreateWithFlags(&S->copy_st, hipStreamNonBlocking));
         S->slots.resize(S->opts.streams);
         S->inflight.resize(S->opts.streams);
+        S->inflight_work.assign(S->opts.streams, 0.0);
         for (auto &sl : S->slots) {
             HIPCHK(hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking));
             HIPCHK(hipMalloc((void **)&sl.sc, 2 * sizeof(unsigned long long)));
@@ -965,6 +967,7 @@ int ctd_hip_residues(ctd_hip_engine *E, const ctd_job *job, const uint64_t *prim
         const bool fast = !(flags & CTD_RUN_GENERIC_ONLY) && Fast<0>::fits(pmax);
         J.flags = (J.fam == FAM_SMARANDACHE && sm_narrow_ok(J.param, J.n1, pmax)) ? JOB_SM_NARROW : 0u;
         if (pmax < FP_PMAX && !(flags & CTD_RUN_NO_FP64)) J.flags |= JOB_FP64;
+        if (flags & CTD_RUN_SM_FULL_STEP) J.flags |= JOB_SM_FULL_STEP;
         const uint64_t items = (nprimes + 63) & ~63ull;
         launch_tf_any(J, E->d_primes, nprimes, nullptr, fast, dim3((uint32_t)div_up(items, 256)), dim3(256),
                       E->stream);

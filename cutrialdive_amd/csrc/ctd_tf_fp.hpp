@@ -34,12 +34,12 @@
 //
 // Per index this costs (VALU instructions, compare included):
 //
-//   doubling x = 2x           1.19 (p < 2^37), 1.375 (p < 2^45), 1.75
+//   doubling x = 2x           1.25 (p < 2^37), 1.44 (p < 2^45), 1.81
 //                                  (integer W64 form: 7)
 //   fixed n  x = x + c        1.25 (6)
-//   fibonacci / lucas         5   (6)
+//   fibonacci / lucas         2.09 theta stride form, 5 pair form (6)
 //   tribonacci                5.5 (11.25)
-//   smarandache r m + n + 1   7   (12.25), for p > 8 n1
+//   smarandache r m + n + 1   7   (12.25), for p > 8 n1; 5 when p (0.75 m + 1) < 2^53
 //
 // and the per-prime set-up (powers, doubling formulas, Smarandache jump) runs
 // on 6-instruction multiplies instead of ~40-instruction Montgomery products.
@@ -50,6 +50,8 @@
 namespace ctd {
 
 constexpr uint64_t FP_PMAX = 1ull << 48;   // every prime of the wave must be below
+// the Smarandache short form needs p (0.75 m + 1) below 2^53 (margin for the rounded product)
+constexpr double SM_SHORT_LIMIT = 9007199254740992.0 - 1073741824.0;
 
 struct Fmod {
     double p, invp;
@@ -428,6 +430,12 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
         // differ by 2^(52-e) in the pattern (and the carry into the exponent at
         // 2^(e+1) still gives the exact value), so an integer add steps it and
         // the VALU does 7 instructions per index, the compare included.
+        //
+        // Short form, for a segment where every lane has p (0.75 m + 1) < 2^53:
+        // then |k p| < 2^53, so w = fma(k + M, -p, M p) = -k p is exact (M the
+        // rounding constant, M p exact for p < 2^50), r m + w = r m - k p is
+        // exact, and r' = (r m - k p) + c takes 4 instructions and the compare
+        // (the same bound |r'| <= 0.75 p holds).
         const uint64_t bb = J.param;
         double r = sm_value_fp(F, bb, ns);
         record_hits(J, idx, p, r == 0.0);
@@ -440,6 +448,7 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
         double m = 1.0;
         for (uint32_t i = 0; i < d; ++i) m *= (double)bb;
         double mp = m * F.invp;
+        const double pM = F.p * Fmod::MAGIC;
         uint64_t n = ns; // index of the current state
         while (n + 1 < ne) {
             // steps n -> n+1 .. seg_last -> seg_last+1 share m = b^digits(n+1)
@@ -451,33 +460,46 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
             const uint32_t cnt = (uint32_t)(seg_last - n + 1); // below 2^32: n1 <= 2^32
             const uint64_t delta = 1ull << (52 - e);
             uint64_t cb = __double_as_longlong((double)c0);
-            auto step = [&](void) {
-                const double k = fma(r, mp, Fmod::MAGIC) - Fmod::MAGIC;
-                const double t = fma(r, m, -(k * ph));
-                r = fma(-k, pl, t + __longlong_as_double(cb));
-                cb += delta;
-            };
-            uint32_t kk = 0;
-            for (; kk + UNROLL <= cnt; kk += UNROLL) {
-                step(); const uint64_t m0 = __ballot(r == 0.0);
-                step(); const uint64_t m1 = __ballot(r == 0.0);
-                step(); const uint64_t m2 = __ballot(r == 0.0);
-                step(); const uint64_t m3 = __ballot(r == 0.0);
-                if (m0 | m1 | m2 | m3) {
-                    const uint32_t lane = __lane_id();
-                    const uint64_t at = hit_base(idx, kk);
-                    if (m0) record_hits_mask(J, at + 1, p, (m0 >> lane) & 1, m0);
-                    if (m1) record_hits_mask(J, at + 2, p, (m1 >> lane) & 1, m1);
-                    if (m2) record_hits_mask(J, at + 3, p, (m2 >> lane) & 1, m2);
-                    if (m3) record_hits_mask(J, at + 4, p, (m3 >> lane) & 1, m3);
+            // one wave-uniform choice per segment (m is fixed inside it)
+            const bool short_form =
+                !(J.flags & JOB_SM_FULL_STEP) && __ballot(!(F.p * (0.75 * m + 1.0) < SM_SHORT_LIMIT)) == 0;
+            auto run = [&](auto step) {
+                uint32_t kk = 0;
+                for (; kk + UNROLL <= cnt; kk += UNROLL) {
+                    step(); const uint64_t m0 = __ballot(r == 0.0);
+                    step(); const uint64_t m1 = __ballot(r == 0.0);
+                    step(); const uint64_t m2 = __ballot(r == 0.0);
+                    step(); const uint64_t m3 = __ballot(r == 0.0);
+                    if (m0 | m1 | m2 | m3) {
+                        const uint32_t lane = __lane_id();
+                        const uint64_t at = hit_base(idx, kk);
+                        if (m0) record_hits_mask(J, at + 1, p, (m0 >> lane) & 1, m0);
+                        if (m1) record_hits_mask(J, at + 2, p, (m1 >> lane) & 1, m1);
+                        if (m2) record_hits_mask(J, at + 3, p, (m2 >> lane) & 1, m2);
+                        if (m3) record_hits_mask(J, at + 4, p, (m3 >> lane) & 1, m3);
+                    }
                 }
-            }
-            idx += kk;
-            for (; kk < cnt; ++kk) {
-                step();
-                ++idx;
-                const uint64_t mk = __ballot(r == 0.0);
-                if (mk) record_hits_mask(J, idx, p, (mk >> __lane_id()) & 1, mk);
+                idx += kk;
+                for (; kk < cnt; ++kk) {
+                    step();
+                    ++idx;
+                    const uint64_t mk = __ballot(r == 0.0);
+                    if (mk) record_hits_mask(J, idx, p, (mk >> __lane_id()) & 1, mk);
+                }
+            };
+            if (short_form) {
+                run([&](void) {
+                    const double k1 = fma(r, mp, Fmod::MAGIC);
+                    r = fma(r, m, fma(k1, -F.p, pM)) + __longlong_as_double(cb);
+                    cb += delta;
+                });
+            } else {
+                run([&](void) {
+                    const double k = fma(r, mp, Fmod::MAGIC) - Fmod::MAGIC;
+                    const double t = fma(r, m, -(k * ph));
+                    r = fma(-k, pl, t + __longlong_as_double(cb));
+                    cb += delta;
+                });
             }
             n = seg_last + 1;
             if (n + 1 == nxt) { // the next appended number has one more digit

@@ -83,6 +83,7 @@ typedef struct ctd_run_opts {
 
 #define CTD_RUN_GENERIC_ONLY 1u /* force the generic 64-bit Montgomery kernels */
 #define CTD_RUN_NO_FP64 2u      /* integer forms only (no FP64 residues below 2^48) */
+#define CTD_RUN_SM_FULL_STEP 4u /* Smarandache FP64 form: the general 7-instruction step only (no short step) */
 
 typedef struct ctd_hip_engine ctd_hip_engine;
 
@@ -241,7 +242,7 @@ int ctd_hip_server_close(ctd_hip_server *srv);
 /* Test hook of the residue kernels: for each odd modulus primes[i] (host
  * array), runs the same kernel path as ctd_hip_trial_factor over [n0, n1) and
  * writes the plain residue S(n1-1) mod primes[i] to out[i] (host array).
- * flags takes CTD_RUN_GENERIC_ONLY.  Hits are recorded into scratch buffers
+ * flags takes the CTD_RUN_* values.  Hits are recorded into scratch buffers
  * and discarded.  No reference counterpart: it exposes the per-prime value
  * that trial_div_by_odd_primes (trial_factoring.cuh:156-228) keeps in
  * registers, so every step of the propagation can be checked, not only the

@@ -288,3 +288,18 @@ def test_theta_form_windows(engine, seq, n0, n1, f0, f1, chunk):
     kw = {"chunk_length": chunk} if chunk else {}
     g, gc, _ = gpu_lists(engine, seq, 0, n0, n1, f0, f1, **kw)
     assert_same(g, gc, seq, 0, n0, n1, f0, f1)
+
+
+@pytest.mark.parametrize("base", [10, 2, 16])
+def test_smarandache_step_forms_with_hits(engine, base):
+    # primes in [2^20, 2^21) are above 8 n1, so the FP64 form runs; p m is far
+    # below 2^53, so it takes the short step by default, and
+    # CTD_RUN_SM_FULL_STEP runs the general step on the same primes.  About
+    # 250 hits per base.
+    n0, n1 = 1, 5000
+    f0, f1 = 1 << 20, 1 << 21
+    r = o.trial_factor(o.SMARANDACHE, base, n0, n1, f0, f1, threads=16)
+    assert sum(len(x) for x in r) > 100
+    for flags in (0, nat.CTD_RUN_SM_FULL_STEP):
+        g, _, _ = gpu_lists(engine, o.SMARANDACHE, base, n0, n1, f0, f1, flags=flags)
+        assert g == r, flags

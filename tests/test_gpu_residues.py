@@ -87,3 +87,50 @@ def test_smarandache_base10_digit_boundaries(engine):
 def test_residues_rejects_even_moduli(engine):
     with pytest.raises(RuntimeError):
         engine.residues(o.MERSENNE, 0, 1, 10, [4], 0)
+
+
+def _digits_multiplier(base, n):
+    m = base
+    while m <= n:
+        m *= base
+    return m
+
+
+@pytest.mark.parametrize("base", [10, 2, 16, 7])
+def test_smarandache_short_step_limit(engine, base):
+    # The short Smarandache step needs p (0.75 m + 1) < 2^53 for the digit
+    # multiplier m of the segment; a wave takes it only when all its lanes
+    # qualify.  For each multiplier of the run: a wave of primes just below
+    # the limit, a wave just above it, and waves that interleave primes below
+    # the limit with primes at 2-4 times it (where the short step would be
+    # inexact).  Final residues are compared with and without the short step.
+    n0, n1 = 5, 2100
+    mults = sorted({_digits_multiplier(base, n) for n in range(n0, n1)})
+    fp_max = (1 << 48) - 1
+    rng = random.Random(base)
+    for m in mults:
+        lim = int((2 ** 53 - 2 ** 30) / (0.75 * m + 1))
+        if lim >= fp_max or lim < 16 * n1:
+            continue
+        below, above, far = [], [], []
+        p = lim
+        while len(below) < 96:
+            p = int(sympy.prevprime(p))
+            below.append(p)
+        p = lim
+        while len(above) < 64 and p < fp_max:
+            p = int(sympy.nextprime(p))
+            if p < fp_max:
+                above.append(p)
+        far_hi = min(4 * lim, fp_max - 1000)
+        while 2 * lim < far_hi and len(far) < 32:
+            q = int(sympy.nextprime(rng.randrange(2 * lim, far_hi)))
+            if q < fp_max:
+                far.append(q)
+        mixed = [x for pair in zip(below[64:], far) for x in pair] if far else below[64:]
+        primes = below[:64] + above + mixed
+        want = o.residues_last(o.SMARANDACHE, base, n0, n1 - n0, primes)
+        for flags in (0, nat.CTD_RUN_SM_FULL_STEP):
+            got = engine.residues(o.SMARANDACHE, base, n0, n1, primes, flags)
+            bad = np.nonzero(got != want)[0]
+            assert len(bad) == 0, (m, flags, [(primes[i], int(got[i]), int(want[i])) for i in bad[:5]])
