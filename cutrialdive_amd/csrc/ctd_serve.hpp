@@ -10,11 +10,14 @@
 //           budget (the first job is always admitted) and the streams need
 //           work: allocate the device buffers, copy the host buffers in, run
 //           the mod-2 kernel, and cut the prime window into tiles of about
-//           2^33 updates (at most tile_length integers);
-//   issue   each stream keeps at most SERVE_DEPTH tiles in flight; a free place goes
-//           to the oldest admitted job that still has tiles, so one job can
-//           spread over every stream and jobs that arrive later join as soon
-//           as the earlier ones have issued all their tiles;
+//           2^35 VALU lane-instructions (at most tile_length integers);
+//   issue   each stream keeps up to SERVE_STREAM_WORK of estimated work in
+//           flight (at least one tile, at most SERVE_DEPTH): enough that it
+//           never runs dry between scheduler passes, little enough that a
+//           new job does not queue behind much committed work.  A free place
+//           goes to the oldest admitted job that still has tiles, so one job
+//           can spread over every stream and jobs that arrive later join as
+//           soon as the earlier ones have issued all their tiles;
 //   retire  a tile is done when its event has completed; a job whose tiles are
 //           all done is copied back to its host buffers on the copy stream and
 //           completes when that copy has, which frees its device memory.
@@ -29,10 +32,12 @@
 // kernels, the same tiles, only the interleaving differs.
 
 constexpr uint64_t SERVE_TILE_DEFAULT = 1ull << 26;
-constexpr double SERVE_TILE_UPDATES = 8589934592.0; // 2^33 updates: ~1 ms of the whole device
+// work = estimated updates x the family's VALU instructions per update
+constexpr double SERVE_TILE_WORK = 34359738368.0; // 2^35 lane-instructions: ~0.9 ms of the whole device
 constexpr uint64_t SERVE_BUDGET_DEFAULT = 8ull << 30;
 constexpr uint32_t SERVE_QUEUE_DEFAULT = 65536;
-constexpr uint32_t SERVE_DEPTH = 4; // tiles in flight per stream
+constexpr uint32_t SERVE_DEPTH = 8; // at most this many tiles in flight per stream
+constexpr double SERVE_STREAM_WORK = 2.0 * SERVE_TILE_WORK; // estimated work in flight per stream
 
 // Device buffers of finished jobs are kept for later ones, by power-of-two
 // size class: hipFree can wait for the whole device, which would serialize
@@ -132,6 +137,7 @@ struct Ticket {
 struct InFlight {
     hipEvent_t ev;
     ServeJob *job;
+    double work; // estimated work of the tile (tile_work)
 };
 
 struct ctd_hip_server_impl {
@@ -139,6 +145,7 @@ struct ctd_hip_server_impl {
     ctd_server_opts opts{};
     std::vector<BatchSlot> slots;
     std::vector<std::deque<InFlight>> inflight;
+    std::vector<double> inflight_work; // per stream, sum of InFlight::work
     hipStream_t copy_st = nullptr;
     std::vector<hipEvent_t> event_pool;
     DevicePool pool;
@@ -283,11 +290,11 @@ bool admit(ctd_hip_server_impl *S, ServeJob *J)
     }
     J->ready = pool_event(S);
     HIPCHK(hipEventRecord(J->ready, S->copy_st));
-    // tiles of about SERVE_TILE_UPDATES updates (at most tile_length
-    // integers, at least one segment): a job with many indices is cut finer,
+    // tiles of about SERVE_TILE_WORK (at most tile_length integers, at least
+    // one segment): a job with many indices or a costly family is cut finer,
     // so its tiles spread over every stream instead of holding one
     const double primes_per_int = 1.0 / std::log((double)std::max<uint64_t>(q.f1, 16));
-    const double want = SERVE_TILE_UPDATES / ((double)N * primes_per_int); // integers
+    const double want = SERVE_TILE_WORK / ((double)N * J->plan.ops * primes_per_int); // integers
     const uint64_t segs = std::max<uint64_t>(1, (uint64_t)(want / (double)SEG_SPAN));
     const uint64_t tl = std::min<uint64_t>(S->opts.tile_length, segs * SEG_SPAN);
     if (J->plan.f0 < J->plan.f1) {
@@ -306,6 +313,13 @@ bool admit(ctd_hip_server_impl *S, ServeJob *J)
     }
     S->active.push_back(J);
     return true;
+}
+
+// Estimated work of a tile: primes in [blo, bhi) x the job's indices x the
+// family's VALU instructions per update.
+double tile_work(const JobPlan &P, uint64_t blo, uint64_t bhi)
+{
+    return (double)(bhi - blo) / std::log((double)std::max<uint64_t>(bhi, 16)) * (double)P.N * P.ops;
 }
 
 // One pass of the scheduler; returns true when it made progress.
@@ -343,13 +357,16 @@ bool schedule_once(ctd_hip_server_impl *S)
             if (e != hipSuccess) throw HipError{e, "tile event"};
             S->event_pool.push_back(q.front().ev);
             q.front().job->outstanding--;
+            S->inflight_work[s] -= q.front().work;
             q.pop_front();
             progress = true;
         }
+        if (q.empty()) S->inflight_work[s] = 0.0; // no rounding drift
     }
     // issue
     for (size_t s = 0; s < S->slots.size(); ++s) {
-        while (S->inflight[s].size() < SERVE_DEPTH) {
+        while (S->inflight[s].empty() ||
+               (S->inflight[s].size() < SERVE_DEPTH && S->inflight_work[s] < SERVE_STREAM_WORK)) {
             ServeJob *J = nullptr;
             for (ServeJob *a : S->active)
                 if (!a->tiles.empty()) {
@@ -364,7 +381,9 @@ bool schedule_once(ctd_hip_server_impl *S)
             issue_tile(E, J->plan, sl, T.blo, T.bhi, T.start, J->d_acc, true);
             hipEvent_t ev = pool_event(S);
             HIPCHK(hipEventRecord(ev, sl.st));
-            S->inflight[s].push_back(InFlight{ev, J});
+            const double work = tile_work(J->plan, T.blo, T.bhi);
+            S->inflight[s].push_back(InFlight{ev, J, work});
+            S->inflight_work[s] += work;
             J->outstanding++;
             {
                 std::lock_guard<std::mutex> lk(S->mu);
