@@ -186,22 +186,23 @@ __device__ __forceinline__ uint64_t hit_base(uint64_t idx, uint32_t k)
         }                                                                                          \
     }
 
-// Sixteen indices per scalar branch, for the stride forms whose VALU work per
-// index is one compare: 15 mask ORs and 3 loop instructions on the scalar
-// unit against 19-22 VALU instructions.  STEP16 sets the masks mk[0..15].
-#define CTD_UNROLLED_LOOP_MASKS16(STEP16, STEP1, MASK1)                                          \
+// B indices per scalar branch (B = 12 or 16), for the stride forms whose
+// VALU work per index is one compare: B - 1 mask ORs and 3 loop instructions
+// on the scalar unit against B + 3 to B + 6 VALU instructions.  STEPB sets the
+// masks mk[0..B-1].
+#define CTD_UNROLLED_LOOP_MASKSB(B, STEPB, STEP1, MASK1)                                          \
     {                                                                                              \
         const uint32_t steps32 = (uint32_t)steps;                                                 \
         uint32_t k = 0;                                                                            \
-        for (; k + 16 <= steps32; k += 16) {                                                       \
-            uint64_t mk[16];                                                                       \
-            STEP16;                                                                                \
-            const uint64_t lo8 = (mk[0] | mk[1] | mk[2] | mk[3]) | (mk[4] | mk[5] | mk[6] | mk[7]);   \
-            const uint64_t hi8 = (mk[8] | mk[9] | mk[10] | mk[11]) | (mk[12] | mk[13] | mk[14] | mk[15]); \
-            if (lo8 | hi8) {                                                                       \
+        for (; k + (B) <= steps32; k += (B)) {                                                     \
+            uint64_t mk[B];                                                                        \
+            STEPB;                                                                                 \
+            uint64_t any = 0;                                                                      \
+            _Pragma("unroll") for (int j = 0; j < (B); ++j) any |= mk[j];                         \
+            if (any) {                                                                             \
                 const uint32_t lane = __lane_id();                                                 \
                 const uint64_t at = hit_base(idx, k);                                              \
-                for (int j = 0; j < 16; ++j)                                                       \
+                for (int j = 0; j < (B); ++j)                                                      \
                     if (mk[j]) record_hits_mask(J, at + 1 + j, p, (mk[j] >> lane) & 1, mk[j]);     \
             }                                                                                      \
         }                                                                                          \
@@ -213,6 +214,7 @@ __device__ __forceinline__ uint64_t hit_base(uint64_t idx, uint32_t k)
             if (m1) record_hits_mask(J, idx, p, (m1 >> __lane_id()) & 1, m1);                      \
         }                                                                                          \
     }
+#define CTD_UNROLLED_LOOP_MASKS16(STEP16, STEP1, MASK1) CTD_UNROLLED_LOOP_MASKSB(16, STEP16, STEP1, MASK1)
 
 } // namespace ctd
 

@@ -30,15 +30,15 @@
 //   fixed n   x_{n+j} = x_n + j c: hit at n+j  <=>  x_n == -j c,
 //             x <- red(x + S c)  (4 instructions).
 // |x 2^S / p| < 2^(S-1), so the doubling stride is exact for S = 16 when
-// p < 2^37, S = 8 when p < 2^45 and S = 4 when p < 2^49.
+// p < 2^37, S = 12 when p < 2^41, S = 8 when p < 2^45 and S = 4 when p < 2^49.
 //
 // Per index this costs (VALU instructions, compare included):
 //
-//   doubling x = 2x           1.25 (p < 2^37), 1.44 (p < 2^45), 1.81
-//                                  (integer W64 form: 7)
+//   doubling x = 2x           1.19 (p < 2^37), 1.25 (p < 2^41), 1.375 (p < 2^45),
+//                             1.75 (integer W64 form: 7)
 //   fixed n  x = x + c        1.25 (6)
-//   fibonacci / lucas         2.09 theta stride form, 5 pair form (6)
-//   tribonacci                5.5 (11.25)
+//   fibonacci / lucas         2.03 theta stride form, 5 pair form (6)
+//   tribonacci                4.875 (p < 2^44), 5.5 (11.25)
 //   smarandache r m + n + 1   7   (12.25), for p > 8 n1; 5 when p (0.75 m + 1) < 2^53
 //
 // and the per-prime set-up (powers, doubling formulas, Smarandache jump) runs
@@ -155,10 +155,25 @@ __device__ inline void trib_triple_fp(const Fmod &F, uint64_t n, double &ta, dou
 
 // Sm_b(n) mod p by affine block jumps (the same decomposition as sm_value in
 // ctd_families.hpp): appending t consecutive d-digit numbers m0..m0+t-1 maps S
-// to B^t S + m0 U_t + V_t.
+// to B^t S + m0 U_t + V_t.  Below SM_DIRECT_MAX the n steps themselves are
+// cheaper (about 10 instructions each against about 60 per bit of t).
+constexpr uint64_t SM_DIRECT_MAX = 32;
+
 __device__ inline double sm_value_fp(const Fmod &F, uint64_t b, uint64_t n)
 {
     if (n == 0) return 0.0;
+    if (n < SM_DIRECT_MAX) {
+        double S = 0.0, BF = F.of(b);
+        uint64_t nxt = b; // the first number with one more digit
+        for (uint64_t i = 1; i <= n; ++i) {
+            if (i == nxt) {
+                BF = F.mul(BF, (double)b);
+                nxt *= b;
+            }
+            S = F.red(F.mul(S, BF) + (double)i);
+        }
+        return S;
+    }
     const double bd = (double)b;
     double S = 0.0;
     uint64_t m0 = 1;
@@ -236,7 +251,60 @@ __device__ inline Rphi rphi_theta_pow(const Fmod &F, uint64_t e)
 // Indices per reduction of the theta stride form, and the shortest work item
 // that takes it (its set-up is about 40 ring products).
 constexpr int THETA_STRIDE = 32;
+// primes below which the Tribonacci lazy form keeps 8 unreduced sums exact
+constexpr uint64_t TRIB_LAZY_PMAX = 1ull << 44;
 constexpr uint64_t THETA_MIN_STEPS = 1024;
+
+// v in a vector register, opaque to the compiler: an FMA with it stays a
+// three-operand VALU instruction (a literal operand would force the
+// two-operand form, and a scalar-register copy would be rematerialised inside
+// the loop by scalar instructions).
+__device__ __forceinline__ double opaque_vgpr(double v)
+{
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
+// nb blocks of NS doubling strides of S indices (x advances 2^S per stride;
+// T[j] = t 2^-(j+1)).  The scalar unit issues the OR of every compare's wave
+// mask and the loop control for the four SIMDs of a CU, so with NS S + 3 NS
+// VALU against about NS S + 5 scalar instructions per block, blocks of 48-64
+// indices keep the loop VALU-bound.  Only the OR of the masks is kept: a
+// block with a hit is replayed from its first state to record each hit.
+// x 2^S - p round(x 2^S / p) is computed with the subtracted multiple
+// -(q p - M p) (q the biased quotient) as a negated addend, so the FMA
+// writes x's own register (no copy per stride).
+template <int S, int NS>
+__device__ __forceinline__ double doubling_blocks(const JobArgs &J, uint64_t p, uint64_t &idx, uint32_t nb,
+                                                  double x, const double (&T)[16], double ip, double c,
+                                                  double P, double pM)
+{
+    for (uint32_t i = 0; i < nb; ++i) {
+        const double x0 = x;
+        uint64_t any = 0;
+        _Pragma("unroll")
+        for (int h = 0; h < NS; ++h) {
+            _Pragma("unroll")
+            for (int j = 0; j < S; ++j) any |= __ballot(x == T[j]);
+            x = fma(x, c, -fma(fma(x, ip, Fmod::MAGIC), P, -pM));
+        }
+        if (any) {
+            const uint32_t lane = __lane_id();
+            const uint64_t at = hit_base(idx, i * (uint32_t)(S * NS));
+            double y = x0;
+            for (int h = 0; h < NS; ++h) {
+                _Pragma("unroll")
+                for (int j = 0; j < S; ++j) {
+                    const uint64_t m = __ballot(y == T[j]);
+                    if (m) record_hits_mask(J, at + 1 + (uint32_t)(h * S + j), p, (m >> lane) & 1, m);
+                }
+                y = fma(y, c, -fma(fma(y, ip, Fmod::MAGIC), P, -pM));
+            }
+        }
+    }
+    idx += (uint64_t)nb * (S * NS);
+    return x;
+}
 
 // The FP64 form of Fast<FAM>::run for a wave whose primes are all below
 // FP_PMAX (and, for smarandache, above 8 n1).  Returns the plain
@@ -259,41 +327,42 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
         T[0] = F.red(tgt * inv2);
         _Pragma("unroll")
         for (int j = 1; j < 16; ++j) T[j] = F.red(F.mul(T[j - 1], inv2));
-        const double mp = -F.p, pM = F.p * Fmod::MAGIC;
+        const double pM = F.p * Fmod::MAGIC;
+        uint32_t left = (uint32_t)steps;
+        auto singles = [&](void) {
+            for (; left; --left) {
+                x = F.red(x + x);
+                ++idx;
+                const uint64_t m = __ballot(x == tgt);
+                if (m) record_hits_mask(J, idx, p, (m >> __lane_id()) & 1, m);
+            }
+        };
         if (__ballot(p >= (1ull << 37)) == 0) {
-            const double ip = F.invp * 65536.0;
-            CTD_UNROLLED_LOOP_MASKS16(
-                {
-                    _Pragma("unroll")
-                    for (int j = 0; j < 16; ++j) mk[j] = __ballot(x == T[j]);
-                    x = fma(x, 65536.0, fma(fma(x, ip, Fmod::MAGIC), mp, pM));
-                },
-                x = F.red(x + x), __ballot(x == tgt))
+            const double ip = F.invp * 65536.0, c = opaque_vgpr(65536.0);
+            x = doubling_blocks<16, 4>(J, p, idx, left / 64, x, T, ip, c, F.p, pM);
+            left %= 64;
+            x = doubling_blocks<16, 1>(J, p, idx, left / 16, x, T, ip, c, F.p, pM);
+            left %= 16;
+        } else if (__ballot(p >= (1ull << 41)) == 0) {
+            const double ip = F.invp * 4096.0, c = opaque_vgpr(4096.0);
+            x = doubling_blocks<12, 4>(J, p, idx, left / 48, x, T, ip, c, F.p, pM);
+            left %= 48;
+            x = doubling_blocks<12, 1>(J, p, idx, left / 12, x, T, ip, c, F.p, pM);
+            left %= 12;
         } else if (__ballot(p >= (1ull << 45)) == 0) {
-            const double ip = F.invp * 256.0;
-            CTD_UNROLLED_LOOP_MASKS16(
-                {
-                    _Pragma("unroll")
-                    for (int h = 0; h < 16; h += 8) {
-                        _Pragma("unroll")
-                        for (int j = 0; j < 8; ++j) mk[h + j] = __ballot(x == T[j]);
-                        x = fma(x, 256.0, fma(fma(x, ip, Fmod::MAGIC), mp, pM));
-                    }
-                },
-                x = F.red(x + x), __ballot(x == tgt))
+            const double ip = F.invp * 256.0, c = opaque_vgpr(256.0);
+            x = doubling_blocks<8, 2>(J, p, idx, left / 16, x, T, ip, c, F.p, pM);
+            left %= 16;
+            x = doubling_blocks<8, 1>(J, p, idx, left / 8, x, T, ip, c, F.p, pM);
+            left %= 8;
         } else {
-            const double ip = F.invp * 16.0;
-            CTD_UNROLLED_LOOP_MASKS16(
-                {
-                    _Pragma("unroll")
-                    for (int h = 0; h < 16; h += 4) {
-                        _Pragma("unroll")
-                        for (int j = 0; j < 4; ++j) mk[h + j] = __ballot(x == T[j]);
-                        x = fma(x, 16.0, fma(fma(x, ip, Fmod::MAGIC), mp, pM));
-                    }
-                },
-                x = F.red(x + x), __ballot(x == tgt))
+            const double ip = F.invp * 16.0, c = opaque_vgpr(16.0);
+            x = doubling_blocks<4, 4>(J, p, idx, left / 16, x, T, ip, c, F.p, pM);
+            left %= 16;
+            x = doubling_blocks<4, 1>(J, p, idx, left / 4, x, T, ip, c, F.p, pM);
+            left %= 4;
         }
+        singles();
         return F.canon(FAM == FAM_PROTH ? x + 1.0 : x - 1.0, p);
     } else if constexpr (FAM == FAM_PROTH_FIXED_N || FAM == FAM_RIESEL_FIXED_N) {
         // x = S(k) = k 2^n +- 1; step x += c with c = 2^n; hit x == 0
@@ -398,6 +467,46 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
         double a, b, c;
         trib_triple_fp(F, ns, a, b, c);
         record_hits(J, idx, p, a == 0.0);
+        if (__ballot(p >= TRIB_LAZY_PMAX) == 0) {
+            // Lazy form, 8 indices per block (39 instructions): from the
+            // reduced state (T(n), T(n+1), T(n+2)), |.| <= 0.75 p, the sums
+            // x[j] = T(n+3+j) are formed unreduced (|x[7]| <= 144.75 p < 2^51.2
+            // for p < 2^44, so every sum is exact).  Each x[j] is tested
+            // without reducing it: with q the biased rounded quotient,
+            // w = fma(q, -p, M p) = -round(x/p) p exactly (|w| < 2^52), and
+            // x = 0 mod p <=> x == -w.  Only the new state x[5..7] is reduced
+            // (x + w, |.| <= p/2 + tiny).  T(n+1) and T(n+2) are reduced already
+            // and compare with 0.
+            const double pM = F.p * Fmod::MAGIC;
+            CTD_UNROLLED_LOOP_MASKSB(8,
+                {
+                    double x[8];
+                    double w[8];
+                    double s = b + c;
+                    x[0] = a + s;
+                    x[1] = s + x[0];
+                    s = x[0] + x[1];
+                    x[2] = c + s;
+                    x[3] = s + x[2];
+                    s = x[2] + x[3];
+                    x[4] = x[1] + s;
+                    x[5] = s + x[4];
+                    s = x[4] + x[5];
+                    x[6] = x[3] + s;
+                    x[7] = s + x[6];
+                    mk[0] = __ballot(b == 0.0);
+                    mk[1] = __ballot(c == 0.0);
+                    _Pragma("unroll")
+                    for (int j = 0; j < 8; ++j) w[j] = fma(fma(x[j], F.invp, Fmod::MAGIC), -F.p, pM);
+                    _Pragma("unroll")
+                    for (int j = 0; j < 6; ++j) mk[2 + j] = __ballot(x[j] == -w[j]);
+                    a = x[5] + w[5];
+                    b = x[6] + w[6];
+                    c = x[7] + w[7];
+                },
+                { const double t = F.red(a + b + c); a = b; b = c; c = t; }, __ballot(a == 0.0))
+            return F.canon(a, p);
+        }
         // two steps share one sum: with s = T(n+1) + T(n+2),
         // T(n+3) = T(n) + s and T(n+4) = s + T(n+3) (11 instructions per 2)
         CTD_UNROLLED_LOOP(

@@ -215,14 +215,15 @@ def test_small_batches_and_resume_accumulate(engine):
 
 
 # Windows for the stride forms of the doubling and fixed-n families: below
-# 2^37 (16 indices per reduction), below 2^45 (8), below 2^48 (4), and two
-# windows straddling 2^37 and 2^45 so one wave holds primes of both ranges.
-STRIDE_WINDOWS = [(1 << 36) + 24690, (1 << 37) - 2000, (1 << 44) + (1 << 40), (1 << 45) - 2000,
-                  (1 << 47) + (1 << 30)]
+# 2^37 (16 indices per reduction), below 2^41 (12), below 2^45 (8), below
+# 2^48 (4), and windows straddling 2^37, 2^41 and 2^45 so one wave holds
+# primes of both ranges.
+STRIDE_WINDOWS = [(1 << 36) + 24690, (1 << 37) - 2000, (1 << 40) + (1 << 39), (1 << 41) - 2000,
+                  (1 << 44) + (1 << 40), (1 << 45) - 2000, (1 << 47) + (1 << 30)]
 # Offsets of the planted hit from n0: the check before the loop, every
-# position of a 16-index block, and the single-step tail (N = 600: blocks
-# cover offsets 1..592, the tail 593..599).
-STRIDE_OFFSETS = [0, 1, 2, 5, 15, 16, 17, 300, 592, 593, 599]
+# position of a 12- and a 16-index block, and the single-step tails (N = 600:
+# 16-index blocks cover offsets 1..592, 12-index blocks 1..588).
+STRIDE_OFFSETS = [0, 1, 2, 5, 11, 12, 13, 15, 16, 17, 300, 588, 589, 592, 593, 599]
 
 
 @pytest.mark.parametrize("seq", [o.PROTH, o.RIESEL, o.PROTH_FIXED_N, o.RIESEL_FIXED_N])

@@ -38,10 +38,11 @@ FAMILIES = [
 ]
 
 # prime-size classes: 32-bit residues, 2^31..2^32, the narrow Smarandache
-# range (FP64 Smarandache from 2^35), the config-3/4 range, both sides of the
-# FP64 limit 2^48, and near 2^63 / 2^64 (generic only above 2^63)
-WINDOWS = [(3, 1 << 20), (1 << 31, 1 << 32), (1 << 32, 1 << 36), (1 << 38, 1 << 43), (1 << 47, 1 << 48),
-           (1 << 48, 1 << 49), (1 << 62, 1 << 63), ((1 << 63) + 1, (1 << 64) - 1)]
+# range (FP64 Smarandache from 2^35), the 12-index doubling stride (2^37..2^41),
+# the config-3/4 range, both sides of the FP64 limit 2^48, and near 2^63 /
+# 2^64 (generic only above 2^63)
+WINDOWS = [(3, 1 << 20), (1 << 31, 1 << 32), (1 << 32, 1 << 36), (1 << 37, 1 << 41), (1 << 38, 1 << 43),
+           (1 << 47, 1 << 48), (1 << 48, 1 << 49), (1 << 62, 1 << 63), ((1 << 63) + 1, (1 << 64) - 1)]
 
 
 def random_primes(lo, hi, k, seed):
@@ -134,3 +135,25 @@ def test_smarandache_short_step_limit(engine, base):
             got = engine.residues(o.SMARANDACHE, base, n0, n1, primes, flags)
             bad = np.nonzero(got != want)[0]
             assert len(bad) == 0, (m, flags, [(primes[i], int(got[i]), int(want[i])) for i in bad[:5]])
+
+
+def test_tribonacci_lazy_form_limit(engine):
+    # The 8-index lazy Tribonacci form runs on waves whose primes are all
+    # below 2^44; a wave of primes just below, one just above, and waves that
+    # interleave both (which take the pair form).
+    lim = 1 << 44
+    below, above = [], []
+    p = lim
+    while len(below) < 96:
+        p = int(sympy.prevprime(p))
+        below.append(p)
+    p = lim
+    while len(above) < 96:
+        p = int(sympy.nextprime(p))
+        above.append(p)
+    mixed = [x for pair in zip(below[64:], above[64:]) for x in pair]
+    primes = below[:64] + above[:64] + mixed
+    for n0, n1 in ((5, 2100), (1, 9), (3, 11), (1000, 1017)):
+        want = o.residues_last(o.TRIBONACCI, 0, n0, n1 - n0, primes)
+        got = engine.residues(o.TRIBONACCI, 0, n0, n1, primes, 0)
+        assert np.array_equal(got, want), (n0, n1)
