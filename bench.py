@@ -119,7 +119,7 @@ def window_ops(t, fam: str, lo: int, hi: int, n_items: int = 0, sm=None) -> floa
     """Prime-count-weighted loop ops per update over the window [lo, hi).
     sm = (base, n0, n1) for a Smarandache job: its FP64 form takes the short
     step on the primes below 2^53 / (0.75 m + 1) of each multiplier m."""
-    cuts = {lo, hi} | {1 << b for b in (31, 35, 37, 45, 48) if lo < (1 << b) < hi}
+    cuts = {lo, hi} | {1 << b for b in (31, 35, 37, 41, 44, 45, 48) if lo < (1 << b) < hi}
     mults = sm_multipliers(*sm) if (fam == "smarandache" and sm) else []
     lims = {m: int(SM_SHORT_LIMIT / (0.75 * m + 1)) for m, _ in mults}
     cuts |= {x for x in lims.values() if lo < x < hi}
@@ -409,7 +409,8 @@ def run_serve(args, torch, ctd, nat):
     def serve(jobs, arrivals):
         """Submits jobs at the arrival times (None: all at once); returns
         (latencies ms, digests, seconds from first submit to last completion, stats)."""
-        srv = ctd.Server(eng, tile_length=1 << 24, streams=args.serve_streams, fresh_buffers=True)
+        srv = ctd.Server(eng, tile_length=1 << 24, streams=args.serve_streams, fresh_buffers=True,
+                         fifo=args.serve_fifo)
         lat = [0.0] * len(jobs)
         dig = [None] * len(jobs)
         tickets = [None] * len(jobs)
@@ -461,6 +462,17 @@ def run_serve(args, torch, ctd, nat):
     print(f"# batched form on the same jobs: {ncap / batch_s:.1f} jobs/s ({bst.updates / batch_s:.3e} updates/s)",
           file=sys.stderr, flush=True)
 
+    # estimated VALU work of each job (primes x indices x loop ops): latency
+    # is also reported separately for the 10 % of jobs with the most work
+    t_ops = ops_table()
+    work = np.array([(c.f1 - c.f0) / math.log(c.f1) * (c.n1 - c.n0)
+                     * window_ops(t_ops, FAMILY_NAMES[c.seq_id], c.f0, c.f1, c.n1 - c.n0, (c.param, c.n0, c.n1))
+                     for c in cjobs])
+    large = work >= np.percentile(work, 90)
+
+    def pct(x):
+        return {"p50_ms": float(np.percentile(x, 50)), "p99_ms": float(np.percentile(x, 99))}
+
     runs = []
     for L in loads:
         arr = workloads.poisson_arrivals(len(cjobs), L * capacity, seed=workloads.SEED + int(L * 1000))
@@ -468,10 +480,15 @@ def run_serve(args, torch, ctd, nat):
         lat_np = np.array(lat)
         runs.append({"load": L, "offered_jobs_per_s": L * capacity, "jobs": len(cjobs), "seconds": el,
                      "p50_ms": float(np.percentile(lat_np, 50)), "p99_ms": float(np.percentile(lat_np, 99)),
-                     "mean_ms": float(lat_np.mean()), "updates_per_s": st.updates / el, "updates": st.updates,
+                     "mean_ms": float(lat_np.mean()),
+                     "by_work": {"largest_10pct": pct(lat_np[large]), "other_90pct": pct(lat_np[~large])},
+                     "updates_per_s": st.updates / el, "updates": st.updates,
                      "tiles": st.tiles, "active_max": st.active_max, "pending_max": st.pending_max,
                      "digests": dig})
-        print(f"# load {L}: {el:.1f} s, p50 {runs[-1]['p50_ms']:.1f} ms, p99 {runs[-1]['p99_ms']:.1f} ms, "
+        bw = runs[-1]["by_work"]
+        print(f"# load {L}: {el:.1f} s, p50 {runs[-1]['p50_ms']:.1f} ms, p99 {runs[-1]['p99_ms']:.1f} ms "
+              f"(largest 10 %: {bw['largest_10pct']['p50_ms']:.1f} / {bw['largest_10pct']['p99_ms']:.1f}, "
+              f"others: {bw['other_90pct']['p50_ms']:.1f} / {bw['other_90pct']['p99_ms']:.1f}), "
               f"{st.updates / el:.3e} updates/s, active max {st.active_max}", file=sys.stderr, flush=True)
 
     # plain form, one job per call: the reference the served buffers are compared with
@@ -535,6 +552,7 @@ def run_serve(args, torch, ctd, nat):
             "batched_form_same_updates_per_s": bst.updates / batch_s,
             "tile_length": 1 << 24,
             "streams": args.serve_streams,
+            "policy": "fifo" if args.serve_fifo else "alternately the oldest job and the least remaining work",
         },
         "serving": runs,
         "plain_check_seconds": plain_s,
@@ -714,6 +732,8 @@ def main() -> int:
     ap.add_argument("--loads", default="0.5,0.7,0.9", help="serve: offered loads (fractions of capacity)")
     ap.add_argument("--capacity-jobs", type=int, default=2000, help="serve: jobs of the capacity measurement")
     ap.add_argument("--serve-streams", type=int, default=4, help="serve: streams (resident prime slots)")
+    ap.add_argument("--serve-fifo", action="store_true",
+                    help="serve: tiles go to the oldest job (default: alternately the oldest and the least remaining work)")
     ap.add_argument("--serve-cpu-jobs", type=int, default=12, help="serve: served jobs checked against the oracle")
     ap.add_argument("--jobs", type=int, default=256, help="mixed: jobs drawn from the seeded generator")
     ap.add_argument("--tile-bits", type=int, default=0, help="mixed: log2 of the tile length (0 = engine default)")

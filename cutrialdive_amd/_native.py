@@ -31,6 +31,7 @@ CTD_BATCH_NO_GRAPH = 2
 CTD_BATCH_NO_AUTOTUNE = 4
 CTD_BATCH_NO_FP64 = 8
 CTD_SERVE_FRESH_BUFFERS = 1
+CTD_SERVE_FIFO = 2
 
 
 class CtdJob(ctypes.Structure):

@@ -14,10 +14,11 @@
 //   issue   each stream keeps up to SERVE_STREAM_WORK of estimated work in
 //           flight (at least one tile, at most SERVE_DEPTH): enough that it
 //           never runs dry between scheduler passes, little enough that a
-//           new job does not queue behind much committed work.  A free place
-//           goes to the oldest admitted job that still has tiles, so one job
-//           can spread over every stream and jobs that arrive later join as
-//           soon as the earlier ones have issued all their tiles;
+//           new job does not queue behind much committed work.  Free places
+//           alternate between the oldest admitted job that still has tiles
+//           and the one with the least remaining work (pick_job; with
+//           CTD_SERVE_FIFO always the oldest); one job can spread over every
+//           stream;
 //   retire  a tile is done when its event has completed; a job whose tiles are
 //           all done is copied back to its host buffers on the copy stream and
 //           completes when that copy has, which frees its device memory.
@@ -126,6 +127,7 @@ struct ServeJob {
     hipEvent_t done = nullptr;
     bool copying = false;
     std::chrono::steady_clock::time_point t_submit;
+    double work_left = 0; // estimated work of the tiles not yet issued
 };
 
 struct Ticket {
@@ -146,6 +148,7 @@ struct ctd_hip_server_impl {
     std::vector<BatchSlot> slots;
     std::vector<std::deque<InFlight>> inflight;
     std::vector<double> inflight_work; // per stream, sum of InFlight::work
+    bool pick_least = false;           // pick_job: the next place goes to the least remaining work
     hipStream_t copy_st = nullptr;
     std::vector<hipEvent_t> event_pool;
     DevicePool pool;
@@ -229,6 +232,13 @@ void deliver(ctd_hip_server_impl *S, Ticket &t, std::unique_lock<std::mutex> &lk
 
 // Admission of one job (scheduler thread).  Returns false when the job has to
 // wait for memory.
+// Estimated work of a tile: primes in [blo, bhi) x the job's indices x the
+// family's VALU instructions per update.
+double tile_work(const JobPlan &P, uint64_t blo, uint64_t bhi)
+{
+    return (double)(bhi - blo) / std::log((double)std::max<uint64_t>(bhi, 16)) * (double)P.N * P.ops;
+}
+
 bool admit(ctd_hip_server_impl *S, ServeJob *J)
 {
     ctd_hip_engine *E = S->E;
@@ -302,6 +312,7 @@ bool admit(ctd_hip_server_impl *S, ServeJob *J)
         for (uint64_t blo = lo0; blo < J->plan.f1; blo += tl) {
             const uint64_t bhi = (J->plan.f1 - blo > tl) ? blo + tl : J->plan.f1;
             J->tiles.push_back(Tile{0, blo, bhi, std::max(blo, J->plan.f0), 0.0, 0u, false});
+            J->work_left += tile_work(J->plan, blo, bhi);
             if (bhi == J->plan.f1) break;
         }
     }
@@ -315,11 +326,23 @@ bool admit(ctd_hip_server_impl *S, ServeJob *J)
     return true;
 }
 
-// Estimated work of a tile: primes in [blo, bhi) x the job's indices x the
-// family's VALU instructions per update.
-double tile_work(const JobPlan &P, uint64_t blo, uint64_t bhi)
+// The admitted job that gets the next free place in a stream.  Default:
+// places alternate between the oldest job that still has tiles and the job
+// with the least remaining work, so a short job that arrives behind long
+// ones starts at the next free place, while the long ones keep every other
+// place (no job waits for ever, and the streams keep a mix of long tiles
+// that fill the device and short ones).  CTD_SERVE_FIFO: always the oldest.
+ServeJob *pick_job(ctd_hip_server_impl *S)
 {
-    return (double)(bhi - blo) / std::log((double)std::max<uint64_t>(bhi, 16)) * (double)P.N * P.ops;
+    ServeJob *oldest = nullptr, *least = nullptr;
+    for (ServeJob *a : S->active) {
+        if (a->tiles.empty()) continue;
+        if (!oldest) oldest = a;
+        if (!least || a->work_left < least->work_left) least = a;
+    }
+    if (S->opts.flags & CTD_SERVE_FIFO) return oldest;
+    S->pick_least = !S->pick_least;
+    return S->pick_least ? least : oldest;
 }
 
 // One pass of the scheduler; returns true when it made progress.
@@ -367,12 +390,7 @@ bool schedule_once(ctd_hip_server_impl *S)
     for (size_t s = 0; s < S->slots.size(); ++s) {
         while (S->inflight[s].empty() ||
                (S->inflight[s].size() < SERVE_DEPTH && S->inflight_work[s] < SERVE_STREAM_WORK)) {
-            ServeJob *J = nullptr;
-            for (ServeJob *a : S->active)
-                if (!a->tiles.empty()) {
-                    J = a;
-                    break;
-                }
+            ServeJob *J = pick_job(S);
             if (!J) break;
             Tile T = J->tiles.front();
             J->tiles.pop_front();
@@ -382,6 +400,7 @@ bool schedule_once(ctd_hip_server_impl *S)
             hipEvent_t ev = pool_event(S);
             HIPCHK(hipEventRecord(ev, sl.st));
             const double work = tile_work(J->plan, T.blo, T.bhi);
+            J->work_left = J->tiles.empty() ? 0.0 : J->work_left - work;
             S->inflight[s].push_back(InFlight{ev, J, work});
             S->inflight_work[s] += work;
             J->outstanding++;

@@ -208,6 +208,9 @@ typedef struct ctd_server_opts {
 } ctd_server_opts;
 #define CTD_SERVE_FRESH_BUFFERS 1u /* jobs start from empty factors buffers: the host
                                       buffers are only written, never read */
+#define CTD_SERVE_FIFO 2u          /* tiles go to the oldest admitted job (default: places
+                                      alternate between the oldest job and the one with the
+                                      least remaining work) */
 
 typedef struct ctd_server_stats {
     uint64_t submitted, admitted, completed, failed;

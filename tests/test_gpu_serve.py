@@ -158,3 +158,22 @@ def test_poll_delivers_and_close_collects(engine):
     for s, (c, f) in zip(jobs[1:], bufs[1:]):
         rc, rf = oracle.trial_factor_raw(s.seq_id, s.param, s.n0, s.n1, s.f0, s.f1, 64)
         assert_same(c, f, rc, rf, 64, "close")
+
+
+def test_fifo_policy_gives_the_same_buffers(engine):
+    # the tile order (by default alternately the oldest job and the least
+    # remaining work, CTD_SERVE_FIFO oldest first) changes only the
+    # interleaving, never a job's factors
+    jobs = small_jobs(30, seed=7, window_bits=22, nmax=3000)
+    out = {}
+    for fifo in (False, True):
+        print("fifo", fifo, flush=True)
+        srv = ctd.Server(engine, tile_length=1 << 20, streams=4, fifo=fifo)
+        tickets = [srv.submit(job_of(s)) for s in jobs]
+        out[fifo] = [srv.wait(t)[:2] for t in tickets]
+        srv.close()
+    for s, (c0, f0), (c1, f1) in zip(jobs, out[False], out[True]):
+        assert_same(c0, f0, c1, f1, 64, s)
+    for s, (c, f) in list(zip(jobs, out[False]))[:8]:
+        rc, rf = oracle.trial_factor_raw(s.seq_id, s.param, s.n0, s.n1, s.f0, s.f1, 64)
+        assert_same(c, f, rc, rf, 64, s)
