@@ -248,8 +248,11 @@ __device__ inline Rphi rphi_theta_pow(const Fmod &F, uint64_t e)
     return x;
 }
 
-// Indices per reduction of the theta stride form, and the shortest work item
-// that takes it (its set-up is about 40 ring products).
+// Indices per reduction of the theta stride form (32 compares against 33
+// VALU instructions of the ring product; 64 would need 128 registers of
+// targets and leave 3 waves per SIMD, too few to hide the compares' mask
+// writes), and the shortest work item that takes it (its set-up is about 40
+// ring products).
 constexpr int THETA_STRIDE = 32;
 // primes below which the Tribonacci lazy form keeps 8 unreduced sums exact
 constexpr uint64_t TRIB_LAZY_PMAX = 1ull << 44;
@@ -291,7 +294,7 @@ __device__ __forceinline__ double doubling_blocks(const JobArgs &J, uint64_t p, 
         if (any) {
             const uint32_t lane = __lane_id();
             const uint64_t at = hit_base(idx, i * (uint32_t)(S * NS));
-            double y = x0;
+            double y = opaque_vgpr(x0); // no reuse of the loop's masks (they would stay live)
             for (int h = 0; h < NS; ++h) {
                 _Pragma("unroll")
                 for (int j = 0; j < S; ++j) {
@@ -416,12 +419,14 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
                 _Pragma("unroll")
                 for (int j = 0; j < THETA_STRIDE; ++j) any |= __ballot(x.v == Tv[j]);
                 if (any) {
-                    // confirm each match: y = x theta^(j+1) must equal eps
+                    // replay the block: x theta^(j+1) == eps on both coordinates
+                    // (x is made opaque so that the loop's masks are not kept
+                    // live for this rare path)
                     const uint64_t at = hit_base(idx, k);
-                    Rphi y = x;
+                    Rphi y = {opaque_vgpr(x.u), opaque_vgpr(x.v)};
                     for (int j = 0; j < THETA_STRIDE; ++j) {
                         y = rphi_mul_theta(F, y);
-                        if (__ballot(x.v == Tv[j])) record_hits(J, at + 1 + j, p, y.u == eps && y.v == 0.0);
+                        record_hits(J, at + 1 + j, p, y.u == eps && y.v == 0.0);
                     }
                 }
                 x = rphi_mul_const(F, x, g.u, g.v, gw);
