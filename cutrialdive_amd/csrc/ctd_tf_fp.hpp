@@ -37,7 +37,7 @@
 //   doubling x = 2x           1.19 (p < 2^37), 1.25 (p < 2^41), 1.375 (p < 2^45),
 //                             1.75 (integer W64 form: 7)
 //   fixed n  x = x + c        1.25 (6)
-//   fibonacci / lucas         2.03 theta stride form, 5 pair form (6)
+//   fibonacci / lucas         1.53 theta stride form, 5 pair form (6)
 //   tribonacci                4.875 (p < 2^44), 5.5 (11.25)
 //   smarandache r m + n + 1   7   (12.25), for p > 8 n1; 5 when p (0.75 m + 1) < 2^53
 //
@@ -248,15 +248,15 @@ __device__ inline Rphi rphi_theta_pow(const Fmod &F, uint64_t e)
     return x;
 }
 
-// Indices per reduction of the theta stride form (32 compares against 33
-// VALU instructions of the ring product; 64 would need 128 registers of
-// targets and leave 3 waves per SIMD, too few to hide the compares' mask
-// writes), and the shortest work item that takes it (its set-up is about 40
+// Indices per reduction of the theta stride form (64 compares of 32-bit
+// target words against 33 VALU instructions of the ring product; the 64
+// words take 64 registers, double targets would take 128 and leave 3 waves
+// per SIMD), and the shortest work item that takes it (its set-up is about 60
 // ring products).
-constexpr int THETA_STRIDE = 32;
+constexpr int THETA_STRIDE = 64;
+constexpr uint64_t THETA_MIN_STEPS = 1024;
 // primes below which the Tribonacci lazy form keeps 8 unreduced sums exact
 constexpr uint64_t TRIB_LAZY_PMAX = 1ull << 44;
-constexpr uint64_t THETA_MIN_STEPS = 1024;
 
 // v in a vector register, opaque to the compiler: an FMA with it stays a
 // three-operand VALU instruction (a literal operand would force the
@@ -393,20 +393,30 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
             // Fibonacci, -1 for Lucas).  The THETA_STRIDE indices after x_n
             // are hits when x_n == eps theta^-j.  Every power of theta has
             // norm u^2 + uv - v^2 = 1, so for a given v at most two u qualify
-            // (u and -v - u): the loop compares the v coordinate only, one
-            // compare per index, and a match is confirmed on both coordinates
-            // before it is recorded.  x advances THETA_STRIDE indices by one
+            // (u and -v - u): the loop compares (a 32-bit word of) the v
+            // coordinate only, one compare per index, and a match is
+            // confirmed on both coordinates before it is recorded.  x advances THETA_STRIDE indices by one
             // ring product with theta^THETA_STRIDE.
             const double eps = FAM == FAM_FIBONACCI ? 1.0 : -1.0;
             Rphi x = rphi_theta_pow(F, ns);
             record_hits(J, idx, p, x.u == eps && x.v == 0.0);
-            double Tv[THETA_STRIDE];
+            // targets: the low 32 bits of v + M (M the rounding constant, which
+            // moves an integer |v| < 2^51 into the low bits of the pattern).
+            // Equal v give equal words; a word match that is not a hit (values
+            // 2^32 apart) is rejected by the exact replay.  Two words share a
+            // 64-bit register pair, so the targets take THETA_STRIDE registers.
+            uint64_t Tw[THETA_STRIDE / 2];
             {
                 Rphi t = {eps, 0.0};
                 _Pragma("unroll")
-                for (int j = 0; j < THETA_STRIDE; ++j) {
+                for (int j = 0; j < THETA_STRIDE; j += 2) {
                     t = rphi_mul_theta_inv(F, t);
-                    Tv[j] = t.v;
+                    const uint64_t w0 = (uint32_t)__double_as_longlong(t.v + Fmod::MAGIC);
+                    t = rphi_mul_theta_inv(F, t);
+                    const uint64_t w1 = (uint32_t)__double_as_longlong(t.v + Fmod::MAGIC);
+                    uint64_t pair = w0 | (w1 << 32);
+                    asm volatile("" : "+v"(pair)); // keep the words packed in one register pair
+                    Tw[j / 2] = pair;
                 }
             }
             Rphi g = {-1.0, -1.0}; // theta^THETA_STRIDE by squaring
@@ -415,9 +425,13 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
             const uint32_t steps32 = (uint32_t)steps;
             uint32_t k = 0;
             for (; k + THETA_STRIDE <= steps32; k += THETA_STRIDE) {
+                const uint32_t xw = (uint32_t)__double_as_longlong(x.v + Fmod::MAGIC);
                 uint64_t any = 0;
                 _Pragma("unroll")
-                for (int j = 0; j < THETA_STRIDE; ++j) any |= __ballot(x.v == Tv[j]);
+                for (int j = 0; j < THETA_STRIDE / 2; ++j) {
+                    any |= __ballot(xw == (uint32_t)Tw[j]);
+                    any |= __ballot(xw == (uint32_t)(Tw[j] >> 32));
+                }
                 if (any) {
                     // replay the block: x theta^(j+1) == eps on both coordinates
                     // (x is made opaque so that the loop's masks are not kept
