@@ -282,6 +282,32 @@ def run_mixed(args, torch, ctd, nat):
         result_plain = {"value": p_upd / pdt, "seconds": pdt, "updates": p_upd, "launch_form": "one job per call",
                         "factors_identical_to_batched": bool(match)}
         print(f"# plain form {pdt:.1f} s, identical factors: {match}", file=sys.stderr, flush=True)
+        # the plain form with the reference kernel's arithmetic shape: one
+        # 64-bit modular multiply-reduce per update (CTD_RUN_GENERIC_ONLY),
+        # one job per call, on the first 2^generic_sample_bits integers of each
+        # window (a bounded sample: this form is an order of magnitude slower)
+        if args.generic_sample_bits:
+            opts = nat.CtdRunOpts(0, 0, 0, 0, nat.CTD_RUN_GENERIC_ONLY)
+            d_counts.zero_()
+            d_factors.zero_()
+            torch.cuda.synchronize()
+            g_upd = 0
+            t0 = time.perf_counter()
+            for k, c in enumerate(cjobs):
+                w = nat.CtdJob(c.seq_id, c.max_factors, c.param, c.n0, c.n1, c.f0,
+                               min(c.f1, c.f0 + (1 << args.generic_sample_bits)))
+                eng.run_job_device(w, cptr[k], fptr[k], opts=opts)
+                g_upd += eng.last_stats().updates
+            torch.cuda.synchronize()
+            gdt = time.perf_counter() - t0
+            result_plain["generic_sample"] = {
+                "value": g_upd / gdt, "seconds": gdt, "updates": g_upd,
+                "form": "plain, one job per call, one 64-bit Montgomery multiply-reduce per update "
+                        "(CTD_RUN_GENERIC_ONLY), device sieve",
+                "sample": f"first 2^{args.generic_sample_bits} integers of each of the {len(cjobs)} windows"}
+            result_plain["batched_vs_generic_plain"] = value / (g_upd / gdt)
+            print(f"# generic plain form on the sample: {g_upd / gdt:.3e} updates/s ({gdt:.1f} s)", file=sys.stderr,
+                  flush=True)
 
     c_upd, c_src = mixed_ops(jobs, full_batch=args.family is None and args.jobs == 256)
     peak, peak_add = probe_peak(eng)
@@ -742,6 +768,9 @@ def main() -> int:
     ap.add_argument("--no-graph", action="store_true", help="mixed: issue the tiles eagerly (profilers)")
     ap.add_argument("--family", default=None, help="mixed: keep only the jobs of one family (per-family rates)")
     ap.add_argument("--no-plain", action="store_true", help="mixed: skip the plain-form comparison")
+    ap.add_argument("--generic-sample-bits", type=int, default=24,
+                    help="mixed: also time the plain form with per-update 64-bit multiply-reduce kernels on the "
+                         "first 2^B integers of each window (0: skip)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-sample-bits", type=int, default=None)
     ap.add_argument("--max-factors", type=int, default=1024)

@@ -36,7 +36,7 @@
 //
 //   doubling x = 2x           1.19 (p < 2^37), 1.25 (p < 2^41), 1.375 (p < 2^45),
 //                             1.75 (integer W64 form: 7)
-//   fixed n  x = x + c        1.25 (6)
+//   fixed n  x = x + c        1.08 (6)
 //   fibonacci / lucas         1.53 theta stride form, 5 pair form (6)
 //   tribonacci                4.875 (p < 2^44), 5.5 (11.25)
 //   smarandache r m + n + 1   7   (12.25), for p > 8 n1; 5 when p (0.75 m + 1) < 2^53
@@ -309,6 +309,38 @@ __device__ __forceinline__ double doubling_blocks(const JobArgs &J, uint64_t p, 
     return x;
 }
 
+// nb blocks of B fixed-n indices (x advances c per index, cB = B c per
+// block; Tw packs the words of the targets -(j+1) c).  A canonical x equal to
+// a target has the target's word; a word match is replayed one index at a
+// time, exactly.
+template <int B>
+__device__ __forceinline__ double fixedn_blocks(const JobArgs &J, uint64_t p, uint64_t &idx, uint32_t nb, double x,
+                                                const uint64_t (&Tw)[32], const Fmod &F, double c, double cB)
+{
+    for (uint32_t i = 0; i < nb; ++i) {
+        const uint32_t xw = (uint32_t)__double_as_longlong(x + Fmod::MAGIC);
+        uint64_t any = 0;
+        _Pragma("unroll")
+        for (int j = 0; j < B / 2; ++j) {
+            any |= __ballot(xw == (uint32_t)Tw[j]);
+            any |= __ballot(xw == (uint32_t)(Tw[j] >> 32));
+        }
+        if (any) {
+            const uint32_t lane = __lane_id();
+            const uint64_t at = hit_base(idx, i * (uint32_t)B);
+            double y = opaque_vgpr(x);
+            for (int j = 0; j < B; ++j) {
+                y = F.red(y + c);
+                const uint64_t m = __ballot(y == 0.0);
+                if (m) record_hits_mask(J, at + 1 + (uint32_t)j, p, (m >> lane) & 1, m);
+            }
+        }
+        x = F.red(x + cB);
+    }
+    idx += (uint64_t)nb * B;
+    return x;
+}
+
 // The FP64 form of Fast<FAM>::run for a wave whose primes are all below
 // FP_PMAX (and, for smarandache, above 8 n1).  Returns the plain
 // residue S(ne-1) mod p for the test hook.
@@ -373,19 +405,36 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
         const uint64_t k = ns < (1ull << 52) ? ns : ns % p;
         double x = F.red(F.mul(F.of(k), c) + (FAM == FAM_PROTH_FIXED_N ? 1.0 : -1.0));
         record_hits(J, idx, p, x == 0.0);
-        // T[j] = -(j+1) c, reduced; the state advances by 16 c
-        double T[16];
-        T[0] = -c;
-        _Pragma("unroll")
-        for (int j = 1; j < 16; ++j) T[j] = F.red(T[j - 1] - c);
-        const double c16 = F.red(16.0 * c);
-        CTD_UNROLLED_LOOP_MASKS16(
-            {
-                _Pragma("unroll")
-                for (int j = 0; j < 16; ++j) mk[j] = __ballot(x == T[j]);
-                x = F.red(x + c16);
-            },
-            x = F.red(x + c), __ballot(x == 0.0))
+        // T[j] = -(j+1) c (j < 64), reduced and kept as the low 32-bit words
+        // of T + M (M the rounding constant), two per register pair: a block
+        // of 64 indices costs the word of x, 64 compares and the advance by
+        // 64 c; a block with a word match is replayed exactly
+        uint64_t Tw[32];
+        {
+            double t = 0.0;
+            _Pragma("unroll")
+            for (int j = 0; j < 64; j += 2) {
+                t = F.red(t - c);
+                const uint64_t w0 = (uint32_t)__double_as_longlong(t + Fmod::MAGIC);
+                t = F.red(t - c);
+                const uint64_t w1 = (uint32_t)__double_as_longlong(t + Fmod::MAGIC);
+                uint64_t pair = w0 | (w1 << 32);
+                asm volatile("" : "+v"(pair)); // keep the words packed in one register pair
+                Tw[j / 2] = pair;
+            }
+        }
+        const double c16 = F.red(16.0 * c), c64 = F.red(4.0 * c16);
+        uint32_t left = (uint32_t)steps;
+        x = fixedn_blocks<64>(J, p, idx, left / 64, x, Tw, F, c, c64);
+        left %= 64;
+        x = fixedn_blocks<16>(J, p, idx, left / 16, x, Tw, F, c, c16);
+        left %= 16;
+        for (; left; --left) {
+            x = F.red(x + c);
+            ++idx;
+            const uint64_t m = __ballot(x == 0.0);
+            if (m) record_hits_mask(J, idx, p, (m >> __lane_id()) & 1, m);
+        }
         return F.canon(x, p);
     } else if constexpr (FAM == FAM_FIBONACCI || FAM == FAM_LUCAS) {
         if (J.n0 != 0 && steps >= THETA_MIN_STEPS && __ballot(p == 5) == 0) {
