@@ -72,7 +72,7 @@ __global__ void __launch_bounds__(256) tf_generic_kernel(JobArgs J, const uint64
 }
 
 // Fast kernel: primes below 2^63 with the family's narrow step (ctd_tf_fast.hpp).
-template <uint32_t FAM>
+template <uint32_t FAM, bool SHIFT>
 __global__ void __launch_bounds__(256) tf_fast_kernel(JobArgs J, const uint64_t *__restrict__ primes,
                                                        uint64_t nprimes, const unsigned long long *np_dev)
 {
@@ -89,7 +89,7 @@ __global__ void __launch_bounds__(256) tf_fast_kernel(JobArgs J, const uint64_t 
         const uint64_t ns = J.n0 + c * J.chunk;
         uint64_t ne = ns + J.chunk;
         if (ne > J.n1 || ne < ns) ne = J.n1;
-        Fast<FAM>::run(J, p, ns, ne, i);
+        Fast<FAM>::template run<SHIFT>(J, p, ns, ne, i);
     }
 }
 
@@ -408,8 +408,11 @@ template <uint32_t FAM>
 void launch_tf(const JobArgs &J, const uint64_t *primes, uint64_t nprimes, const unsigned long long *np_dev,
                bool fast, dim3 grid, dim3 block, hipStream_t st)
 {
-    if (fast)
-        hipLaunchKernelGGL(tf_fast_kernel<FAM>, grid, block, 0, st, J, primes, nprimes, np_dev);
+    constexpr bool doubling = FAM == FAM_MERSENNE || FAM == FAM_PROTH || FAM == FAM_RIESEL;
+    if (fast && doubling && (J.flags & JOB_P37))
+        hipLaunchKernelGGL((tf_fast_kernel<FAM, doubling>), grid, block, 0, st, J, primes, nprimes, np_dev);
+    else if (fast)
+        hipLaunchKernelGGL((tf_fast_kernel<FAM, false>), grid, block, 0, st, J, primes, nprimes, np_dev);
     else
         hipLaunchKernelGGL(tf_generic_kernel<FAM>, grid, block, 0, st, J, primes, nprimes, np_dev);
 }
@@ -544,6 +547,7 @@ int run_job(ctd_hip_engine *E, const ctd_job *job, const ctd_run_opts *opts, con
                 J.flags = (J.fam == FAM_SMARANDACHE && sm_narrow_ok(J.param, J.n1, bhi - 1)) ? JOB_SM_NARROW : 0u;
                 if (bhi - 1 < FP_PMAX && !(opts && (opts->flags & CTD_RUN_NO_FP64))) J.flags |= JOB_FP64;
                 if (opts && (opts->flags & CTD_RUN_SM_FULL_STEP)) J.flags |= JOB_SM_FULL_STEP;
+                if (bhi - 1 < (1ull << 37)) J.flags |= JOB_P37;
                 uint32_t block = (opts && opts->block_size > 0) ? (uint32_t)opts->block_size : 256u;
                 block = std::max<uint32_t>(64, std::min<uint32_t>(block, 256)) & ~63u;
                 uint64_t items = ((np + 63) & ~63ull) * nch;
@@ -968,6 +972,7 @@ int ctd_hip_residues(ctd_hip_engine *E, const ctd_job *job, const uint64_t *prim
         J.flags = (J.fam == FAM_SMARANDACHE && sm_narrow_ok(J.param, J.n1, pmax)) ? JOB_SM_NARROW : 0u;
         if (pmax < FP_PMAX && !(flags & CTD_RUN_NO_FP64)) J.flags |= JOB_FP64;
         if (flags & CTD_RUN_SM_FULL_STEP) J.flags |= JOB_SM_FULL_STEP;
+        if (pmax < (1ull << 37)) J.flags |= JOB_P37;
         const uint64_t items = (nprimes + 63) & ~63ull;
         launch_tf_any(J, E->d_primes, nprimes, nullptr, fast, dim3((uint32_t)div_up(items, 256)), dim3(256),
                       E->stream);

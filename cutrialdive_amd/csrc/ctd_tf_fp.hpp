@@ -19,23 +19,26 @@
 // reduction the zero test is x == 0 and the doubling families test x == +-1.
 //
 // Stride forms (doubling and fixed-n families).  When the reduction is exact
-// (round(v/p) correct, which holds when |v/p| 2^-53 < 1/(2p): the distance of
-// v/p from a half-integer is at least 1/(2p) for odd p) every class has the
-// single representative |x| <= (p-1)/2.  Then the S indices after the state
-// x_n are tested by S compares of x_n against precomputed targets, and the
-// state advances S indices at once:
+// (round(v/p) correct, which holds when |v/p| 2^-52 < 1/(2p), i.e. |v| < 2^51:
+// the distance of v/p from a half-integer is at least 1/(2p) for odd p) every
+// class has the single representative |x| <= (p-1)/2.  Then the S indices
+// after the state x_n are tested by S compares of x_n against precomputed
+// targets, and the state advances S indices at once:
 //   doubling  x_{n+j} = x_n 2^j:   hit at n+j  <=>  x_n == t 2^-j,
-//             x <- x 2^S - p round(x 2^S / p)  (3 FMAs: the product by 2^S
-//             is exact, and p round() is exact below 2^53);
+//             p < 2^37: S = 16, x <- x 2^S - p round(x 2^S / p) (3 FMAs: the
+//             product by 2^S is exact, p round() is exact below 2^53, and
+//             |x 2^S / p| < 2^(S-1) keeps the quotient exact);
+//             above: S = 64, x <- x (2^S mod p), an exact product
+//             (mulc_canon, 9 instructions);
 //   fixed n   x_{n+j} = x_n + j c: hit at n+j  <=>  x_n == -j c,
 //             x <- red(x + S c)  (4 instructions).
-// |x 2^S / p| < 2^(S-1), so the doubling stride is exact for S = 16 when
-// p < 2^37, S = 12 when p < 2^41, S = 8 when p < 2^45 and S = 4 when p < 2^49.
+// With S = 64 the compares use the low 32-bit words of x + M and of the
+// targets (low_word), two target words per register pair, and a block whose
+// words match is replayed one index at a time with the exact value.
 //
 // Per index this costs (VALU instructions, compare included):
 //
-//   doubling x = 2x           1.19 (p < 2^37), 1.25 (p < 2^41), 1.375 (p < 2^45),
-//                             1.75 (integer W64 form: 7)
+//   doubling x = 2x           1.19 (p < 2^37), 1.16 (integer W64 form: 7)
 //   fixed n  x = x + c        1.08 (6)
 //   fibonacci / lucas         1.53 theta stride form, 5 pair form (6)
 //   tribonacci                4.875 (p < 2^44), 5.5 (11.25)
@@ -258,28 +261,54 @@ constexpr uint64_t THETA_MIN_STEPS = 1024;
 // primes below which the Tribonacci lazy form keeps 8 unreduced sums exact
 constexpr uint64_t TRIB_LAZY_PMAX = 1ull << 44;
 
-// v in a vector register, opaque to the compiler: an FMA with it stays a
-// three-operand VALU instruction (a literal operand would force the
-// two-operand form, and a scalar-register copy would be rematerialised inside
-// the loop by scalar instructions).
+// v in a vector register, opaque to the compiler: a rare replay path that
+// starts from it cannot reuse the values (wave masks) its loop computed from
+// v, which would otherwise stay live across the loop.
 __device__ __forceinline__ double opaque_vgpr(double v)
 {
     asm volatile("" : "+v"(v));
     return v;
 }
 
-// nb blocks of NS doubling strides of S indices (x advances 2^S per stride;
-// T[j] = t 2^-(j+1)).  The scalar unit issues the OR of every compare's wave
-// mask and the loop control for the four SIMDs of a CU, so with NS S + 3 NS
-// VALU against about NS S + 5 scalar instructions per block, blocks of 48-64
-// indices keep the loop VALU-bound.  Only the OR of the masks is kept: a
-// block with a hit is replayed from its first state to record each hit.
-// x 2^S - p round(x 2^S / p) is computed with the subtracted multiple
-// -(q p - M p) (q the biased quotient) as a negated addend, so the FMA
-// writes x's own register (no copy per stride).
+// x C mod p, exact and canonical (|result| <= (p-1)/2), for |x|, |C| <= p/2:
+// hi + lo = x C exactly (Dekker), hi - round(hi/p) p is exact and small, and
+// the final reduction of a value below 2^51 rounds its quotient correctly.
+__device__ __forceinline__ double mulc_canon(const Fmod &F, double x, double C)
+{
+    const double hi = x * C;
+    const double lo = fma(x, C, -hi);
+    return F.red(fma(-F.quot(hi), F.p, hi) + lo);
+}
+
+// t / 2 mod p, canonical, for canonical t: t / 2 when t is even, else
+// (t - sign(t) p) / 2 (exact: f is 0 or 1/2).
+__device__ __forceinline__ double half_canon(const Fmod &F, double t)
+{
+    const double h = 0.5 * t;
+    const double f = h - floor(h);
+    return fma(-f, copysign(F.p, t), h);
+}
+
+// The low 32-bit word of v + M (M the rounding constant): for an integer
+// |v| < 2^51 the sum is exact and its low word is v's low 32 bits, so equal
+// values give equal words.
+__device__ __forceinline__ uint32_t low_word(double v)
+{
+    return (uint32_t)__double_as_longlong(v + Fmod::MAGIC);
+}
+
+// nb blocks of NS doubling strides of S indices for p < 2^(53-S) (x advances
+// 2^S per stride by an exact shift, T[j] = t 2^-(j+1)).  The scalar unit
+// issues the OR of every compare's wave mask and the loop control for the
+// four SIMDs of a CU, so with NS S + 3 NS VALU against about NS S + 5 scalar
+// instructions per block, blocks of 64 indices keep the loop VALU-bound.
+// Only the OR of the masks is kept: a block with a hit is replayed from its
+// first state to record each hit.  x 2^S - p round(x 2^S / p) is computed
+// with the subtracted multiple -(q p - M p) (q the biased quotient) as a
+// negated addend, so the FMA writes x's own register (no copy per stride).
 template <int S, int NS>
 __device__ __forceinline__ double doubling_blocks(const JobArgs &J, uint64_t p, uint64_t &idx, uint32_t nb,
-                                                  double x, const double (&T)[16], double ip, double c,
+                                                  double x, const double (&T)[S], double ip, double c,
                                                   double P, double pM)
 {
     for (uint32_t i = 0; i < nb; ++i) {
@@ -309,6 +338,40 @@ __device__ __forceinline__ double doubling_blocks(const JobArgs &J, uint64_t p, 
     return x;
 }
 
+// nb blocks of B doubling indices.  Tw packs the words of the targets
+// t 2^-(j+1) (j < 64), two per register pair; x advances B indices by one
+// exact product with C = 2^B mod p.  A block costs the word of x, B compares
+// and 9 instructions of product; a block with a word match is replayed one
+// index at a time, exactly.
+template <int B>
+__device__ __forceinline__ double doubling_words(const JobArgs &J, uint64_t p, uint64_t &idx, uint32_t nb,
+                                                 double x, const uint64_t (&Tw)[32], const Fmod &F, double C,
+                                                 double tgt)
+{
+    for (uint32_t i = 0; i < nb; ++i) {
+        const uint32_t xw = low_word(x);
+        uint64_t any = 0;
+        _Pragma("unroll")
+        for (int j = 0; j < B / 2; ++j) {
+            any |= __ballot(xw == (uint32_t)Tw[j]);
+            any |= __ballot(xw == (uint32_t)(Tw[j] >> 32));
+        }
+        if (any) {
+            const uint32_t lane = __lane_id();
+            const uint64_t at = hit_base(idx, i * (uint32_t)B);
+            double y = opaque_vgpr(x); // no reuse of the loop's masks (they would stay live)
+            for (int j = 0; j < B; ++j) {
+                y = F.red(y + y);
+                const uint64_t m = __ballot(y == tgt);
+                if (m) record_hits_mask(J, at + 1 + (uint32_t)j, p, (m >> lane) & 1, m);
+            }
+        }
+        x = mulc_canon(F, x, C);
+    }
+    idx += (uint64_t)nb * B;
+    return x;
+}
+
 // nb blocks of B fixed-n indices (x advances c per index, cB = B c per
 // block; Tw packs the words of the targets -(j+1) c).  A canonical x equal to
 // a target has the target's word; a word match is replayed one index at a
@@ -318,7 +381,7 @@ __device__ __forceinline__ double fixedn_blocks(const JobArgs &J, uint64_t p, ui
                                                 const uint64_t (&Tw)[32], const Fmod &F, double c, double cB)
 {
     for (uint32_t i = 0; i < nb; ++i) {
-        const uint32_t xw = (uint32_t)__double_as_longlong(x + Fmod::MAGIC);
+        const uint32_t xw = low_word(x);
         uint64_t any = 0;
         _Pragma("unroll")
         for (int j = 0; j < B / 2; ++j) {
@@ -344,7 +407,7 @@ __device__ __forceinline__ double fixedn_blocks(const JobArgs &J, uint64_t p, ui
 // The FP64 form of Fast<FAM>::run for a wave whose primes are all below
 // FP_PMAX (and, for smarandache, above 8 n1).  Returns the plain
 // residue S(ne-1) mod p for the test hook.
-template <uint32_t FAM>
+template <uint32_t FAM, bool SHIFT>
 __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t ne, uint64_t idx, uint64_t steps)
 {
     Fmod F;
@@ -356,48 +419,53 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
         x = F.red(x);
         const double tgt = (FAM == FAM_PROTH) ? -1.0 : 1.0;
         record_hits(J, idx, p, x == tgt);
-        // T[j] = t 2^-(j+1), reduced; 2^-1 = (p+1)/2
-        const double inv2 = 0.5 * (F.p + 1.0);
-        double T[16];
-        T[0] = F.red(tgt * inv2);
-        _Pragma("unroll")
-        for (int j = 1; j < 16; ++j) T[j] = F.red(F.mul(T[j - 1], inv2));
-        const double pM = F.p * Fmod::MAGIC;
         uint32_t left = (uint32_t)steps;
-        auto singles = [&](void) {
-            for (; left; --left) {
-                x = F.red(x + x);
-                ++idx;
-                const uint64_t m = __ballot(x == tgt);
-                if (m) record_hits_mask(J, idx, p, (m >> __lane_id()) & 1, m);
-            }
-        };
-        if (__ballot(p >= (1ull << 37)) == 0) {
-            const double ip = F.invp * 65536.0, c = opaque_vgpr(65536.0);
+        // The two forms are separate kernels (SHIFT, chosen by the host when
+        // every prime of the launch is below 2^37): sharing one kernel would
+        // give the shift form the word form's register count and cost it two
+        // waves per SIMD.
+        if constexpr (SHIFT) {
+            // exact shift by 2^16 (3 instructions per 16 indices); a wave with a
+            // prime above 2^37 (not launched by the host) takes single steps
+            if (__ballot(p >= (1ull << 37)) == 0) {
+            double T[16];
+            T[0] = half_canon(F, tgt);
+            _Pragma("unroll")
+            for (int j = 1; j < 16; ++j) T[j] = half_canon(F, T[j - 1]);
+            const double pM = F.p * Fmod::MAGIC, ip = F.invp * 65536.0, c = opaque_vgpr(65536.0);
             x = doubling_blocks<16, 4>(J, p, idx, left / 64, x, T, ip, c, F.p, pM);
             left %= 64;
             x = doubling_blocks<16, 1>(J, p, idx, left / 16, x, T, ip, c, F.p, pM);
             left %= 16;
-        } else if (__ballot(p >= (1ull << 41)) == 0) {
-            const double ip = F.invp * 4096.0, c = opaque_vgpr(4096.0);
-            x = doubling_blocks<12, 4>(J, p, idx, left / 48, x, T, ip, c, F.p, pM);
-            left %= 48;
-            x = doubling_blocks<12, 1>(J, p, idx, left / 12, x, T, ip, c, F.p, pM);
-            left %= 12;
-        } else if (__ballot(p >= (1ull << 45)) == 0) {
-            const double ip = F.invp * 256.0, c = opaque_vgpr(256.0);
-            x = doubling_blocks<8, 2>(J, p, idx, left / 16, x, T, ip, c, F.p, pM);
-            left %= 16;
-            x = doubling_blocks<8, 1>(J, p, idx, left / 8, x, T, ip, c, F.p, pM);
-            left %= 8;
+            }
         } else {
-            const double ip = F.invp * 16.0, c = opaque_vgpr(16.0);
-            x = doubling_blocks<4, 4>(J, p, idx, left / 16, x, T, ip, c, F.p, pM);
+            // targets t 2^-(j+1), j < 64, canonical, as packed words
+            uint64_t Tw[32];
+            {
+                double t = tgt;
+                _Pragma("unroll")
+                for (int j = 0; j < 64; j += 2) {
+                    t = half_canon(F, t);
+                    const uint64_t w0 = low_word(t);
+                    t = half_canon(F, t);
+                    const uint64_t w1 = low_word(t);
+                    uint64_t pair = w0 | (w1 << 32);
+                    asm volatile("" : "+v"(pair)); // keep the words packed in one register pair
+                    Tw[j / 2] = pair;
+                }
+            }
+            const double C16 = F.red(F.pow2(16)), C64 = F.red(F.pow2(64));
+            x = doubling_words<64>(J, p, idx, left / 64, x, Tw, F, C64, tgt);
+            left %= 64;
+            x = doubling_words<16>(J, p, idx, left / 16, x, Tw, F, C16, tgt);
             left %= 16;
-            x = doubling_blocks<4, 1>(J, p, idx, left / 4, x, T, ip, c, F.p, pM);
-            left %= 4;
         }
-        singles();
+        for (; left; --left) {
+            x = F.red(x + x);
+            ++idx;
+            const uint64_t m = __ballot(x == tgt);
+            if (m) record_hits_mask(J, idx, p, (m >> __lane_id()) & 1, m);
+        }
         return F.canon(FAM == FAM_PROTH ? x + 1.0 : x - 1.0, p);
     } else if constexpr (FAM == FAM_PROTH_FIXED_N || FAM == FAM_RIESEL_FIXED_N) {
         // x = S(k) = k 2^n +- 1; step x += c with c = 2^n; hit x == 0
@@ -415,9 +483,9 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
             _Pragma("unroll")
             for (int j = 0; j < 64; j += 2) {
                 t = F.red(t - c);
-                const uint64_t w0 = (uint32_t)__double_as_longlong(t + Fmod::MAGIC);
+                const uint64_t w0 = low_word(t);
                 t = F.red(t - c);
-                const uint64_t w1 = (uint32_t)__double_as_longlong(t + Fmod::MAGIC);
+                const uint64_t w1 = low_word(t);
                 uint64_t pair = w0 | (w1 << 32);
                 asm volatile("" : "+v"(pair)); // keep the words packed in one register pair
                 Tw[j / 2] = pair;
@@ -460,9 +528,9 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
                 _Pragma("unroll")
                 for (int j = 0; j < THETA_STRIDE; j += 2) {
                     t = rphi_mul_theta_inv(F, t);
-                    const uint64_t w0 = (uint32_t)__double_as_longlong(t.v + Fmod::MAGIC);
+                    const uint64_t w0 = low_word(t.v);
                     t = rphi_mul_theta_inv(F, t);
-                    const uint64_t w1 = (uint32_t)__double_as_longlong(t.v + Fmod::MAGIC);
+                    const uint64_t w1 = low_word(t.v);
                     uint64_t pair = w0 | (w1 << 32);
                     asm volatile("" : "+v"(pair)); // keep the words packed in one register pair
                     Tw[j / 2] = pair;
@@ -474,7 +542,7 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
             const uint32_t steps32 = (uint32_t)steps;
             uint32_t k = 0;
             for (; k + THETA_STRIDE <= steps32; k += THETA_STRIDE) {
-                const uint32_t xw = (uint32_t)__double_as_longlong(x.v + Fmod::MAGIC);
+                const uint32_t xw = low_word(x.v);
                 uint64_t any = 0;
                 _Pragma("unroll")
                 for (int j = 0; j < THETA_STRIDE / 2; ++j) {
