@@ -8,7 +8,10 @@ prime window in [2^38, 2^43) each), about 1.3e14 updates.  One step runs the
 whole batch in the batched form (one hipGraph launch, autotuned launch shapes,
 factors buffers resident in HBM).  The same jobs then run in the plain form,
 one ctd_hip_trial_factor_device call per job, for the batched-vs-plain ratio,
-and the two sets of factors buffers are compared.
+and the two sets of factors buffers are compared.  A bounded sample (the first
+2^24 integers of each window) is also timed in the plain form with the
+per-update 64-bit multiply-reduce kernels (CTD_RUN_GENERIC_ONLY), the
+arithmetic shape of the reference's kernel.
 
 Other workloads (--workload): single jobs of the plain form, one step per
 device batch of 2^31 integers of the prime window:
