@@ -760,7 +760,7 @@ def main() -> int:
     ap.add_argument("--serve-jobs", type=int, default=10000, help="serve: jobs per offered load")
     ap.add_argument("--loads", default="0.5,0.7,0.9", help="serve: offered loads (fractions of capacity)")
     ap.add_argument("--capacity-jobs", type=int, default=2000, help="serve: jobs of the capacity measurement")
-    ap.add_argument("--serve-streams", type=int, default=4, help="serve: streams (resident prime slots)")
+    ap.add_argument("--serve-streams", type=int, default=3, help="serve: streams (resident prime slots)")
     ap.add_argument("--serve-fifo", action="store_true",
                     help="serve: tiles go to the oldest job (default: alternately the oldest and the least remaining work)")
     ap.add_argument("--serve-cpu-jobs", type=int, default=12, help="serve: served jobs checked against the oracle")

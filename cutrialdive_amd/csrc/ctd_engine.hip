@@ -752,7 +752,10 @@ int ctd_hip_server_open(ctd_hip_engine *E, const ctd_server_opts *opts, ctd_hip_
     if (!S->opts.memory_budget) S->opts.memory_budget = SERVE_BUDGET_DEFAULT;
     if (!S->opts.tile_length) S->opts.tile_length = SERVE_TILE_DEFAULT;
     S->opts.tile_length = std::max<uint64_t>(div_up(S->opts.tile_length, SEG_SPAN), 1) * SEG_SPAN;
-    if (!S->opts.streams) S->opts.streams = 4;
+    // 3 streams: measured on the 10^4-job Poisson stream, 2 give the most
+    // capacity, 3 nearly as much with the lowest p99; from 4 on, the sieve
+    // workgroups of one stream wait for CUs held by the others' residue kernels
+    if (!S->opts.streams) S->opts.streams = SERVE_STREAMS_DEFAULT;
     S->opts.streams = std::min<uint32_t>(S->opts.streams, 16);
     if (!S->opts.queue_limit) S->opts.queue_limit = SERVE_QUEUE_DEFAULT;
     try {

@@ -38,6 +38,7 @@ constexpr double SERVE_TILE_WORK = 34359738368.0; // 2^35 lane-instructions: ~0.
 constexpr uint64_t SERVE_BUDGET_DEFAULT = 8ull << 30;
 constexpr uint32_t SERVE_QUEUE_DEFAULT = 65536;
 constexpr uint32_t SERVE_DEPTH = 8; // at most this many tiles in flight per stream
+constexpr uint32_t SERVE_STREAMS_DEFAULT = 3;
 constexpr double SERVE_STREAM_WORK = 2.0 * SERVE_TILE_WORK; // estimated work in flight per stream
 
 // Device buffers of finished jobs are kept for later ones, by power-of-two

@@ -201,7 +201,7 @@ typedef struct ctd_hip_server ctd_hip_server;
 typedef struct ctd_server_opts {
     uint64_t memory_budget; /* device bytes for admitted jobs' factors buffers (0 = 8 GiB) */
     uint64_t tile_length;   /* integers of a prime window per tile (0 = 2^26) */
-    uint32_t streams;       /* streams = resident prime slots (0 = 4, at most 16) */
+    uint32_t streams;       /* streams = resident prime slots (0 = 3, at most 16) */
     uint32_t queue_limit;   /* jobs waiting for admission before submit returns CTD_E_BUSY (0 = 65536) */
     uint32_t flags;         /* CTD_SERVE_* */
     uint32_t reserved;
