@@ -249,7 +249,7 @@ def run_mixed(args, torch, ctd, nat):
         if k:
             d_counts.zero_()
             d_factors.zero_()
-        st = eng.run_batch_raw(cjobs, cptr, fptr, flags, args.tile_bits and 1 << args.tile_bits)
+        st = eng.run_batch_raw(cjobs, cptr, fptr, flags, args.tile_bits and 1 << args.tile_bits, args.batch_streams)
         upd += st.updates
         primes += st.primes
         bms += st.total_ms
@@ -766,6 +766,7 @@ def main() -> int:
     ap.add_argument("--serve-cpu-jobs", type=int, default=12, help="serve: served jobs checked against the oracle")
     ap.add_argument("--jobs", type=int, default=256, help="mixed: jobs drawn from the seeded generator")
     ap.add_argument("--tile-bits", type=int, default=0, help="mixed: log2 of the tile length (0 = engine default)")
+    ap.add_argument("--batch-streams", type=int, default=0, help="mixed: streams of the batched form (0 = engine default)")
     ap.add_argument("--window-bits", type=int, default=30, help="mixed: log2 of each job's prime window")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="mixed: issue the tiles eagerly (profilers)")
