@@ -304,3 +304,43 @@ def test_smarandache_step_forms_with_hits(engine, base):
     for flags in (0, nat.CTD_RUN_SM_FULL_STEP):
         g, _, _ = gpu_lists(engine, o.SMARANDACHE, base, n0, n1, f0, f1, flags=flags)
         assert g == r, flags
+
+
+def _canon(v, p):
+    v %= p
+    return v if v <= (p - 1) // 2 else v - p
+
+
+@pytest.mark.parametrize("seq", [o.PROTH, o.PROTH_FIXED_N])
+@pytest.mark.parametrize("j", [0, 5, 63])
+def test_word_filter_collision_is_not_a_hit(engine, seq, j):
+    """The 64-index stride forms compare 32-bit words (the low bits of the
+    residue); a state that differs from a target by exactly 2^32 matches its
+    word without being a hit.  The job is built so that the first block of a
+    prime p0 > 2^37 starts on such a state; the replay must reject it, so the
+    factors equal the oracle's."""
+    # x_n0 = T_j -+ 2^32 for the stride target T_j: equal low 32 bits
+    f0, f1 = (1 << 40) + 12345, (1 << 40) + 12345 + 4096
+    primes = [int(x) for x in o.sieve(f0, f1)]
+    p0 = primes[len(primes) // 2]
+    N = 600
+    if seq == o.PROTH:
+        # state x_n = k 2^n, targets T_j = -2^-(j+1)
+        n0 = 1000
+        t = _canon(-pow(2, -(j + 1), p0), p0)
+        x = t + (1 << 32) if t < 0 else t - (1 << 32)  # same low word, still canonical
+        param = (x * pow(2, -n0, p0)) % p0
+        hit_index = n0 + j + 1
+        assert (param * pow(2, hit_index, p0) + 1) % p0 != 0
+    else:
+        # state x_k = k c + 1 (c = 2^n), targets T_j = -(j+1) c
+        param = 777
+        c = pow(2, param, p0)
+        t = _canon(-(j + 1) * c, p0)
+        x = t + (1 << 32) if t < 0 else t - (1 << 32)  # same low word, still canonical
+        n0 = ((x - 1) * pow(c, -1, p0)) % p0
+        hit_index = n0 + j + 1
+        assert (hit_index * c + 1) % p0 != 0
+    g, gc, _ = gpu_lists(engine, seq, param, n0, n0 + N, f0, f1)
+    assert p0 not in g[hit_index - n0]
+    assert_same(g, gc, seq, param, n0, n0 + N, f0, f1)
