@@ -439,7 +439,7 @@ def run_serve(args, torch, ctd, nat):
         """Submits jobs at the arrival times (None: all at once); returns
         (latencies ms, digests, seconds from first submit to last completion, stats)."""
         srv = ctd.Server(eng, tile_length=1 << 24, streams=args.serve_streams, fresh_buffers=True,
-                         fifo=args.serve_fifo)
+                         fifo=args.serve_fifo, depth=args.serve_depth)
         lat = [0.0] * len(jobs)
         dig = [None] * len(jobs)
         tickets = [None] * len(jobs)
@@ -582,6 +582,7 @@ def run_serve(args, torch, ctd, nat):
             "tile_length": 1 << 24,
             "streams": args.serve_streams,
             "policy": "fifo" if args.serve_fifo else "alternately the oldest job and the least remaining work",
+            "depth": args.serve_depth or 2,
         },
         "serving": runs,
         "plain_check_seconds": plain_s,
@@ -761,6 +762,8 @@ def main() -> int:
     ap.add_argument("--loads", default="0.5,0.7,0.9", help="serve: offered loads (fractions of capacity)")
     ap.add_argument("--capacity-jobs", type=int, default=2000, help="serve: jobs of the capacity measurement")
     ap.add_argument("--serve-streams", type=int, default=3, help="serve: streams (resident prime slots)")
+    ap.add_argument("--serve-depth", type=int, default=0,
+                    help="serve: tiles of work each stream keeps in flight (0 = server default, 2)")
     ap.add_argument("--serve-fifo", action="store_true",
                     help="serve: tiles go to the oldest job (default: alternately the oldest and the least remaining work)")
     ap.add_argument("--serve-cpu-jobs", type=int, default=12, help="serve: served jobs checked against the oracle")

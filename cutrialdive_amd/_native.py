@@ -102,7 +102,7 @@ class CtdBatchStats(ctypes.Structure):
 
 class CtdServerOpts(ctypes.Structure):
     _fields_ = [("memory_budget", u64), ("tile_length", u64), ("streams", u32), ("queue_limit", u32),
-                ("flags", u32), ("reserved", u32)]
+                ("flags", u32), ("depth", u32)]
 
 
 class CtdServerStats(ctypes.Structure):

@@ -756,6 +756,7 @@ int ctd_hip_server_open(ctd_hip_engine *E, const ctd_server_opts *opts, ctd_hip_
     // capacity, 3 nearly as much with the lowest p99; from 4 on, the sieve
     // workgroups of one stream wait for CUs held by the others' residue kernels
     if (!S->opts.streams) S->opts.streams = SERVE_STREAMS_DEFAULT;
+    if (S->opts.depth) S->stream_work = std::min<uint32_t>(S->opts.depth, 16) * SERVE_TILE_WORK;
     S->opts.streams = std::min<uint32_t>(S->opts.streams, 16);
     if (!S->opts.queue_limit) S->opts.queue_limit = SERVE_QUEUE_DEFAULT;
     try {

@@ -150,6 +150,7 @@ struct ctd_hip_server_impl {
     std::vector<std::deque<InFlight>> inflight;
     std::vector<double> inflight_work; // per stream, sum of InFlight::work
     bool pick_least = false;           // pick_job: the next place goes to the least remaining work
+    double stream_work = SERVE_STREAM_WORK; // estimated work in flight per stream (opts.depth tiles)
     hipStream_t copy_st = nullptr;
     std::vector<hipEvent_t> event_pool;
     DevicePool pool;
@@ -390,7 +391,7 @@ bool schedule_once(ctd_hip_server_impl *S)
     // issue
     for (size_t s = 0; s < S->slots.size(); ++s) {
         while (S->inflight[s].empty() ||
-               (S->inflight[s].size() < SERVE_DEPTH && S->inflight_work[s] < SERVE_STREAM_WORK)) {
+               (S->inflight[s].size() < SERVE_DEPTH && S->inflight_work[s] < S->stream_work)) {
             ServeJob *J = pick_job(S);
             if (!J) break;
             Tile T = J->tiles.front();

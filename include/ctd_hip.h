@@ -204,7 +204,8 @@ typedef struct ctd_server_opts {
     uint32_t streams;       /* streams = resident prime slots (0 = 3, at most 16) */
     uint32_t queue_limit;   /* jobs waiting for admission before submit returns CTD_E_BUSY (0 = 65536) */
     uint32_t flags;         /* CTD_SERVE_* */
-    uint32_t reserved;
+    uint32_t depth;         /* estimated work each stream keeps in flight, in tiles
+                               of 2^35 VALU lane-instructions (0 = 2, at most 16) */
 } ctd_server_opts;
 #define CTD_SERVE_FRESH_BUFFERS 1u /* jobs start from empty factors buffers: the host
                                       buffers are only written, never read */
