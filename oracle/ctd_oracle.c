@@ -1,9 +1,10 @@
 /*
  * ctd_oracle.c -- CPU restatement of cutrialdive's trial-factoring path.
  *
- * TEST INFRASTRUCTURE ONLY: the checker for the HIP engine and the CPU
- * baseline timed beside it.  Only tests/, __graft_entry__.smoke() and
- * bench.py's cpu_baseline leg load it.  Nothing on the product path does.
+ * TEST INFRASTRUCTURE ONLY: what the HIP engine's results are compared
+ * against, and the CPU baseline timed beside it.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.  Nothing
+ * on the product path does.
  *
  * Each function restates the reference behaviour it cites (file:line under
  * the reference's src/).  Where the reference calls a third-party library

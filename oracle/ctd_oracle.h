@@ -1,8 +1,8 @@
 /*
  * ctd_oracle.h -- CPU restatement of cutrialdive's trial-factoring path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP engine
- * is compared against, and the CPU baseline timed beside it.  It is never
+ * TEST INFRASTRUCTURE ONLY.  This library is what the HIP engine's results
+ * are compared against, and the CPU baseline timed beside it.  It is never
  * linked into, loaded by or called from the product path (libctd_hip.so, the
  * cutrialdive CLI, the batched engine or the serving loop).
  *

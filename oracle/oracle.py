@@ -1,7 +1,7 @@
 """ctypes binding of the oracle (oracle/libctd_oracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
-bench.py's cpu_baseline leg, as the checker.  The product path never loads it.
+bench.py's cpu_baseline leg, to check results against.  The product path never loads it.
 """
 from __future__ import annotations
 

@@ -1,4 +1,4 @@
-"""Pins the oracle (oracle/ctd_oracle.c) before it is trusted as the checker.
+"""Pins the oracle (oracle/ctd_oracle.c) before the GPU results are checked against it.
 
 Every function is checked against (a) the reference's own known-answer
 vectors (tests/golden/reference_kats.json, transcribed from the reference's
