@@ -454,7 +454,8 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
                     Tw[j / 2] = pair;
                 }
             }
-            const double C16 = F.red(F.pow2(16)), C64 = F.red(F.pow2(64));
+            // 2^16 mod p by one exact reduction, 2^64 mod p by two squarings
+            const double C16 = F.red(65536.0), C32 = mulc_canon(F, C16, C16), C64 = mulc_canon(F, C32, C32);
             x = doubling_words<64>(J, p, idx, left / 64, x, Tw, F, C64, tgt);
             left %= 64;
             x = doubling_words<16>(J, p, idx, left / 16, x, Tw, F, C16, tgt);
