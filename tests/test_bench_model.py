@@ -1,0 +1,55 @@
+"""The VALU-per-update model that bench.py's roofline uses (CPU only).
+
+The model must pick, for each prime window, the loop form the kernels pick:
+the Smarandache short step where p (0.75 m + 1) < 2^53 for the digit
+multiplier m, the shift stride below 2^37 and the word stride above for the
+doubling families, the Tribonacci lazy form below 2^44."""
+import os
+import sys
+
+sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+
+import pytest  # noqa: E402
+
+import bench  # noqa: E402
+
+
+def _digits_multiplier(b, x):
+    m = b
+    while m <= x:
+        m *= b
+    return m
+
+
+def test_sm_multipliers_match_brute_force():
+    for b, n0, n1 in ((10, 10, 1000), (2, 1, 20), (16, 1, 2000), (7, 3, 3000), (10, 1, 2)):
+        want = {}
+        for n in range(n0, n1 - 1):  # steps n -> n+1 append n+1
+            m = _digits_multiplier(b, n + 1)
+            want[m] = want.get(m, 0) + 1
+        got = dict(bench.sm_multipliers(b, n0, n1))
+        assert got == want, (b, n0, n1)
+
+
+def test_smarandache_short_and_long_windows():
+    t = bench.ops_table()
+    short, full = t["loop"]["smarandache"]["fp64_short"], t["loop"]["smarandache"]["fp64"]
+    # cfg2: m <= 1000, p < 2^36: every step short
+    assert bench.window_ops(t, "smarandache", 1 << 32, 1 << 36, 990, (10, 10, 1000)) == pytest.approx(short)
+    # m = 10^5 at p near 2^43: p (0.75 m + 1) > 2^53, every step long
+    assert bench.window_ops(t, "smarandache", 1 << 43, (1 << 43) + (1 << 30), 1000,
+                            (10, 20000, 21000)) == pytest.approx(full)
+    # a window straddling the limit of its single multiplier lies strictly between
+    lim = int(bench.SM_SHORT_LIMIT / (0.75 * 10 ** 4 + 1))
+    mid = bench.window_ops(t, "smarandache", lim - (1 << 30), lim + (1 << 30), 1000, (10, 2000, 3000))
+    assert short < mid < full
+
+
+def test_stride_forms_by_prime_size():
+    t = bench.ops_table()
+    shift, words = 1.1875, 1.15625
+    assert bench.window_ops(t, "proth", 1 << 30, 1 << 31) == shift
+    assert bench.window_ops(t, "proth", 1 << 40, 1 << 41) == words
+    assert bench.window_ops(t, "tribonacci", 1 << 40, 1 << 41) == 4.875
+    assert bench.window_ops(t, "tribonacci", 1 << 45, 1 << 46) == 5.5
+    assert bench.window_ops(t, "fibonacci", 1 << 40, 1 << 41, 5000) == t["loop"]["fibonacci"]["fp64_theta"]
