@@ -125,7 +125,9 @@ JobArgs tile_args(const ctd_hip_engine *E, const JobPlan &P, uint64_t est_np, ui
 {
     JobArgs J = P.J;
     uint64_t nch = 1;
-    const uint64_t target = (uint64_t)E->cu * 4096;
+    // lanes a tile should offer: 2048 per CU (8 waves) is enough with several
+    // streams running at once, and fewer index chunks mean less set-up
+    const uint64_t target = (uint64_t)E->cu * 2048;
     if (est_np < target && P.N > 2048) nch = std::min<uint64_t>(div_up(target, std::max<uint64_t>(est_np, 1)),
                                                                  div_up(P.N, 1024));
     nch = std::min<uint64_t>(std::max<uint64_t>(nch * P.shape.split, div_up(P.N, CHUNK_MAX)), P.N);
