@@ -105,6 +105,13 @@ struct Tile {
     bool first;
 };
 
+// Lanes a tile's residue launch should offer (the index range is split into
+// chunks until it does): the batched graph keeps its streams full with 2048
+// per CU (8 waves), and fewer chunks mean less per-chunk set-up; the serving
+// form, whose streams run dry between scheduler passes, measured best at 4096.
+constexpr uint32_t BATCH_LANES_PER_CU = 2048;
+constexpr uint32_t SERVE_LANES_PER_CU = 4096;
+
 // VALU instructions per update of the family's loop (FP64 forms below 2^48,
 // integer forms above), for ordering tiles and balancing the streams.
 double family_cost(uint32_t fam, bool fast, bool fp64, uint64_t N)
@@ -115,19 +122,20 @@ double family_cost(uint32_t fam, bool fast, bool fp64, uint64_t N)
     case FAM_TRIBONACCI: return fp64 ? 5.5 : 11.25;
     case FAM_FIBONACCI:
     case FAM_LUCAS: return fp64 ? (N >= THETA_MIN_STEPS ? 2.1 : 5.0) : 6.0;
+    // doubling and fixed-n strides: 1.1-1.2 in the loop, but their short work
+    // items pay 64-target set-ups; 1.5 sized the serving tiles best
     default: return fp64 ? 1.5 : 7.0;
     }
 }
 
 // Tile arguments: chunking from the estimated prime count and the shape's
 // split, the Smarandache narrow flag from the tile's largest candidate.
-JobArgs tile_args(const ctd_hip_engine *E, const JobPlan &P, uint64_t est_np, uint64_t pmax)
+JobArgs tile_args(const ctd_hip_engine *E, const JobPlan &P, uint64_t est_np, uint64_t pmax,
+                  uint32_t lanes_per_cu = BATCH_LANES_PER_CU)
 {
     JobArgs J = P.J;
     uint64_t nch = 1;
-    // lanes a tile should offer: 2048 per CU (8 waves) is enough with several
-    // streams running at once, and fewer index chunks mean less set-up
-    const uint64_t target = (uint64_t)E->cu * 2048;
+    const uint64_t target = (uint64_t)E->cu * lanes_per_cu;
     if (est_np < target && P.N > 2048) nch = std::min<uint64_t>(div_up(target, std::max<uint64_t>(est_np, 1)),
                                                                  div_up(P.N, 1024));
     nch = std::min<uint64_t>(std::max<uint64_t>(nch * P.shape.split, div_up(P.N, CHUNK_MAX)), P.N);
@@ -169,7 +177,7 @@ void ensure_scratch(ctd_hip_engine *E, uint64_t n)
 // The operations of one tile on its slot's stream: zero the slot counters,
 // sieve, keep the count (tile_out[0..1]), propagate.
 void issue_tile(ctd_hip_engine *E, const JobPlan &P, BatchSlot &S, uint64_t blo, uint64_t bhi, uint64_t start,
-                unsigned long long *tile_out, bool accumulate = false)
+                unsigned long long *tile_out, bool accumulate = false, uint32_t lanes_per_cu = BATCH_LANES_PER_CU)
 {
     hipStream_t st = S.st;
     SieveArgs A{};
@@ -192,7 +200,7 @@ void issue_tile(ctd_hip_engine *E, const JobPlan &P, BatchSlot &S, uint64_t blo,
     hipLaunchKernelGGL(tile_done_kernel, dim3(1), dim3(64), 0, st, S.sc, S.cap, E->d_err, tile_out,
                        (uint32_t)accumulate);
     const uint64_t est = (uint64_t)est_primes(start, bhi);
-    JobArgs J = tile_args(E, P, est, bhi - 1);
+    JobArgs J = tile_args(E, P, est, bhi - 1, lanes_per_cu);
     const bool fast = Fast<0>::fits(bhi - 1);
     launch_tf_any(J, S.primes, S.cap, S.sc, fast, tile_grid(E, P.shape, est, J.nchunks), dim3(P.shape.block), st);
     HIPCHK(hipGetLastError());

@@ -398,7 +398,7 @@ bool schedule_once(ctd_hip_server_impl *S)
             J->tiles.pop_front();
             BatchSlot &sl = S->slots[s];
             HIPCHK(hipStreamWaitEvent(sl.st, J->ready, 0));
-            issue_tile(E, J->plan, sl, T.blo, T.bhi, T.start, J->d_acc, true);
+            issue_tile(E, J->plan, sl, T.blo, T.bhi, T.start, J->d_acc, true, SERVE_LANES_PER_CU);
             hipEvent_t ev = pool_event(S);
             HIPCHK(hipEventRecord(ev, sl.st));
             const double work = tile_work(J->plan, T.blo, T.bhi);
