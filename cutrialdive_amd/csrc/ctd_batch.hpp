@@ -216,7 +216,7 @@ void ensure_slot_buffers(BatchSlot &s, uint64_t tile_len)
 // Times the candidate shapes on a probe tile of the job and returns the best.
 LaunchShape tune_shape(ctd_hip_engine *E, const JobPlan &P, bool fast)
 {
-    const uint64_t lo = P.f0, hi = std::min<uint64_t>(P.f1, P.f0 + TUNE_WINDOW);
+    const uint64_t lo = P.f0, hi = P.f0 < P.f1 ? block_end(P.f0, TUNE_WINDOW, P.f1) : P.f0;
     LaunchShape best;
     if (lo >= hi) return best;
     const uint64_t blo = lo & ~1ull;
@@ -367,7 +367,7 @@ int run_batch(ctd_hip_engine *E, const ctd_job *jobs, uint32_t njobs, uint32_t *
     ctd_batch_stats bs{};
     const uint32_t flags = opts ? opts->flags : 0u;
     uint64_t tile_len = (opts && opts->tile_length) ? opts->tile_length : BATCH_TILE_DEFAULT;
-    tile_len = std::max<uint64_t>(div_up(tile_len, SEG_SPAN), 1) * SEG_SPAN;
+    tile_len = whole_spans(tile_len, SEG_SPAN, BLOCK_SEGS_MAX);
     uint32_t nst = (opts && opts->streams) ? opts->streams : 4u;
     nst = std::max<uint32_t>(1, std::min<uint32_t>(nst, 16));
 
@@ -420,7 +420,7 @@ int run_batch(ctd_hip_engine *E, const ctd_job *jobs, uint32_t njobs, uint32_t *
         const uint64_t lo0 = P.f0 & ~1ull;
         bool first = true;
         for (uint64_t blo = lo0; blo < P.f1; blo += tile_len) {
-            const uint64_t bhi = (P.f1 - blo > tile_len) ? blo + tile_len : P.f1;
+            const uint64_t bhi = block_end(blo, tile_len, P.f1);
             const uint64_t start = std::max(blo, P.f0);
             const double cost = est_primes(start, bhi) * ((double)P.N * P.ops + 300.0) + 2e6;
             tiles.push_back(Tile{j, blo, bhi, start, cost, 0u, first});

@@ -232,18 +232,13 @@ void ensure_buf(T *&ptr, uint64_t &cap, uint64_t need)
     cap = n;
 }
 
-uint64_t isqrt_u64(uint64_t n)
-{
-    uint64_t r = (uint64_t)std::sqrt((double)n);
-    while (r * r > n) --r;
-    while ((r + 1) * (r + 1) <= n && (r + 1) <= 0xffffffffull) ++r;
-    return r;
-}
-
 uint64_t div_up(uint64_t a, uint64_t b) { return a / b + (a % b != 0); }
 
 // Indices per work item at most: the index loops keep 32-bit counters.
 constexpr uint64_t CHUNK_MAX = 1ull << 30;
+// Segments per device batch or tile at most (2^40 integers): a requested
+// length is rounded to whole segments and clamped to this.
+constexpr uint64_t BLOCK_SEGS_MAX = 1ull << 20;
 
 // Sieves [start, end) into an ordered list (u32 or u64) with the current
 // sieving numbers sp/mu.  Returns the count; allocates *out32 or *out64.
@@ -322,7 +317,7 @@ void make_mu(ctd_hip_engine *E, const uint32_t *sp, uint64_t *mu, uint32_t n)
     HIPCHK(hipGetLastError());
 }
 
-// Odd primes <= lim (lim <= 2^32) as the sieving base.
+// Odd primes <= lim (lim < 2^32) as the sieving base.
 void build_base(ctd_hip_engine *E, uint64_t lim)
 {
     if (lim < 65536) lim = 65536;
@@ -330,7 +325,7 @@ void build_base(ctd_hip_engine *E, uint64_t lim)
     // grow geometrically: consecutive batches of a window raise sqrt(f1) a
     // little each time, and sieving numbers beyond sqrt(end) cost a segment
     // nothing (the per-lane loop stops at the first one past it)
-    if (E->d_sp) lim = std::min<uint64_t>(std::max<uint64_t>(lim, 2 * E->base_lim), 1ull << 32);
+    if (E->d_sp) lim = std::min<uint64_t>(std::max<uint64_t>(lim, 2 * E->base_lim), 0xffffffffull);
     // level 0: odd numbers 3..255 sieve [0, 65537)
     uint32_t *l0 = nullptr;
     uint64_t *m0 = nullptr;
@@ -446,6 +441,14 @@ int run_job(ctd_hip_engine *E, const ctd_job *job, const ctd_run_opts *opts, con
 {
     int code;
     if (!valid_job(E, job, &code)) return code;
+    // --block-size goes to the residue launch as given (trial_factoring.cuh:280-286);
+    // the kernels run whole 64-lane waves, at most 256 lanes per workgroup
+    if (opts && opts->block_size && (opts->block_size < 64 || opts->block_size > 256 || opts->block_size % 64))
+        return fail(E, CTD_E_UNSUPPORTED, "block size " + std::to_string(opts->block_size) +
+                                              " is not supported: the residue kernels take 64, 128, 192 or 256 "
+                                              "threads per workgroup");
+    if (opts && opts->grid_size < 0)
+        return fail(E, CTD_E_INVALID, "grid size " + std::to_string(opts->grid_size) + " is not positive");
     E->stats = ctd_run_stats{};
     if (last_prime) *last_prime = 0;
     if (job->n1 <= job->n0 || job->f1 <= job->f0) return CTD_OK; // trial_factoring_impl.hpp:380-387
@@ -480,11 +483,11 @@ int run_job(ctd_hip_engine *E, const ctd_job *job, const ctd_run_opts *opts, con
         build_base(E, isqrt_u64(f1 - 1));
         const bool generic_only = opts && (opts->flags & CTD_RUN_GENERIC_ONLY);
         const bool fast = !generic_only && Fast<0>::fits(f1 - 1);
-        uint64_t blen = (opts && opts->batch_length) ? opts->batch_length : (1ull << 31);
-        blen = std::max<uint64_t>(div_up(blen, SEG_SPAN), 1) * SEG_SPAN;
+        const uint64_t blen = whole_spans((opts && opts->batch_length) ? opts->batch_length : (1ull << 31), SEG_SPAN,
+                                          BLOCK_SEGS_MAX);
         const uint64_t lo0 = f0 & ~1ull;
         for (uint64_t blo = lo0; blo < f1;) {
-            uint64_t bhi = (f1 - blo > blen) ? blo + blen : f1;
+            const uint64_t bhi = block_end(blo, blen, f1);
             uint64_t bstart = std::max(blo, f0);
             uint64_t len = bhi - blo;
             uint64_t nseg = div_up(len, SEG_SPAN);
@@ -548,8 +551,7 @@ int run_job(ctd_hip_engine *E, const ctd_job *job, const ctd_run_opts *opts, con
                 if (bhi - 1 < FP_PMAX && !(opts && (opts->flags & CTD_RUN_NO_FP64))) J.flags |= JOB_FP64;
                 if (opts && (opts->flags & CTD_RUN_SM_FULL_STEP)) J.flags |= JOB_SM_FULL_STEP;
                 if (bhi - 1 < (1ull << 37)) J.flags |= JOB_P37;
-                uint32_t block = (opts && opts->block_size > 0) ? (uint32_t)opts->block_size : 256u;
-                block = std::max<uint32_t>(64, std::min<uint32_t>(block, 256)) & ~63u;
+                const uint32_t block = (opts && opts->block_size > 0) ? (uint32_t)opts->block_size : 256u;
                 uint64_t items = ((np + 63) & ~63ull) * nch;
                 uint64_t grid = (opts && opts->grid_size > 0) ? (uint64_t)opts->grid_size : div_up(items, block);
                 grid = std::min<uint64_t>(grid, 1ull << 30);
@@ -751,7 +753,7 @@ int ctd_hip_server_open(ctd_hip_engine *E, const ctd_server_opts *opts, ctd_hip_
     if (opts) S->opts = *opts;
     if (!S->opts.memory_budget) S->opts.memory_budget = SERVE_BUDGET_DEFAULT;
     if (!S->opts.tile_length) S->opts.tile_length = SERVE_TILE_DEFAULT;
-    S->opts.tile_length = std::max<uint64_t>(div_up(S->opts.tile_length, SEG_SPAN), 1) * SEG_SPAN;
+    S->opts.tile_length = whole_spans(S->opts.tile_length, SEG_SPAN, BLOCK_SEGS_MAX);
     // 3 streams: measured on the 10^4-job Poisson stream, 2 give the most
     // capacity, 3 nearly as much with the lowest p99; from 4 on, the sieve
     // workgroups of one stream wait for CUs held by the others' residue kernels

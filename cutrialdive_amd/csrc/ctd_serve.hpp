@@ -312,7 +312,7 @@ bool admit(ctd_hip_server_impl *S, ServeJob *J)
     if (J->plan.f0 < J->plan.f1) {
         const uint64_t lo0 = J->plan.f0 & ~1ull;
         for (uint64_t blo = lo0; blo < J->plan.f1; blo += tl) {
-            const uint64_t bhi = (J->plan.f1 - blo > tl) ? blo + tl : J->plan.f1;
+            const uint64_t bhi = block_end(blo, tl, J->plan.f1);
             J->tiles.push_back(Tile{0, blo, bhi, std::max(blo, J->plan.f0), 0.0, 0u, false});
             J->work_left += tile_work(J->plan, blo, bhi);
             if (bhi == J->plan.f1) break;

@@ -26,6 +26,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "ctd_ranges.hpp"
+
 namespace ctd {
 
 constexpr uint32_t SEG_BITS = 1u << 19;            // odd numbers per segment
@@ -162,8 +164,9 @@ __device__ __forceinline__ void sieve_mark(uint32_t *bits, uint32_t *phase, cons
         const uint32_t p = A.sp[A.nwheel + k];
         for (uint32_t i = phase[k] + lane * p; i < SEG_BITS; i += 64 * p) atomicOr(&bits[i >> 5], 1u << (i & 31));
     }
-    // 3. one larger sieving number per lane
-    const uint64_t hi_num = lo + SEG_SPAN - 1;
+    // 3. one larger sieving number per lane (the segment's last candidate
+    //    below end saturates: a segment near 2^64 must not wrap)
+    const uint64_t hi_num = seg_last(lo, SEG_SPAN, A.end);
     for (uint32_t k = A.nsmall + tid; k < A.nsp; k += SIEVE_THREADS) {
         const uint32_t p = A.sp[k];
         if ((uint64_t)p * p > hi_num) break; // ascending: no later number marks here

@@ -143,6 +143,27 @@ def test_family_chunked(engine, seq, param):
     assert g == r
 
 
+@pytest.mark.parametrize("block", [64, 128, 192, 256])
+def test_block_and_grid_size_flags(engine, block):
+    # --block-size / --grid-size reach the residue launch as given
+    # (trial_factoring.cuh:280-286); the kernels are grid-stride loops, so any
+    # grid computes the same factors
+    seq, param, n0, n1, f0, f1 = o.PROTH, 1279, 0, 600, 0, 1 << 20
+    r = o.trial_factor(seq, param, n0, n1, f0, f1, 64)
+    for grid in (0, 1, 7, 5000):
+        opts = nat.CtdRunOpts(0, grid, block, 0, 0)
+        counts, factors, _ = engine.run_job(seq, param, n0, n1, f0, f1, 64, opts=opts)
+        assert ctd.FactoringResults.from_buffer(n0, counts, factors, 64).factors == r, (block, grid)
+
+
+@pytest.mark.parametrize("block", [32, 100, 320, 512, 1024])
+def test_unsupported_block_size_is_an_error(engine, block):
+    # the reference hands any block size to the launch; here the kernels take
+    # whole 64-lane waves up to 256 lanes, and other sizes fail before any work
+    with pytest.raises(RuntimeError, match="block size"):
+        engine.run_job(o.PROTH, 1279, 0, 80, 0, 1 << 16, 8, opts=nat.CtdRunOpts(0, 0, block, 0, 0))
+
+
 def test_reference_tf_cases(engine):
     # tests/tf_tests.cpp:70-109
     g, _, _ = gpu_lists(engine, o.MERSENNE, 0, 1, 101, 0, 1 << 16)

@@ -127,6 +127,32 @@ def test_sieve_bruteforce_windows():
         assert got == want, (a, b)
 
 
+def test_sieve_top_of_64_bit_range():
+    # the ten largest primes below 2^64 are 2^64 - d for d in 59, 83, 95, 179,
+    # 189, 257, 279, 323, 353, 363 (the Prime Pages' table of primes just less
+    # than a power of two); --tf-bits 64 sieves up to f1 = 2^64 - 1
+    top = [(1 << 64) - d for d in (363, 353, 323, 279, 257, 189, 179, 95, 83, 59)]
+    assert o.sieve((1 << 64) - 400, (1 << 64) - 1).tolist() == top
+    a = (1 << 64) - 5000
+    assert o.sieve(a, (1 << 64) - 1).tolist() == list(sympy.primerange(a, (1 << 64) - 1))
+    # 2^64 - 873 = 361373 x 51046... is odd with no prime factor below 8192
+    c = (1 << 64) - 873
+    assert min(sympy.factorint(c)) == 361373 and c not in o.sieve(c - 10, c + 10).tolist()
+
+
+def test_trial_factor_near_2_64_planted_hit():
+    # k = -2^-n mod p makes p divide k 2^n + 1 (Proth); the oracle must list p
+    # at n and, checked by brute force, nothing else in the window
+    p = (1 << 64) - 59
+    n = 1000
+    k = (-pow(2, -n, p)) % p
+    f0, f1 = (1 << 64) - 4000, (1 << 64) - 1
+    got = o.trial_factor(o.PROTH, k, n - 20, n + 20, f0, f1, 8)
+    primes = list(sympy.primerange(f0, f1))
+    want = [[q for q in primes if (k * pow(2, m, q) + 1) % q == 0] for m in range(n - 20, n + 20)]
+    assert got == want and got[20] == [p]
+
+
 def test_prime_count_2_30_and_readme_primes():
     assert o.prime_count(0, 1 << 30) == KATS["prime_counts"]["30"]
     assert int(o.sieve((1 << 37) - 200, 1 << 37)[-1]) == KATS["readme_largest_prime_below_2_37"]
@@ -177,7 +203,7 @@ FAMS = [(o.MERSENNE, 0), (o.PROTH, 3), (o.PROTH, 1279), (o.RIESEL, 37), (o.RIESE
 def test_residues_bruteforce(seq, param):
     rng = random.Random(seq * 1000 + param)
     for _ in range(6):
-        d = int(sympy.randprime(3, 1 << rng.choice([8, 20, 33, 50, 63])))
+        d = int(sympy.randprime(3, 1 << rng.choice([8, 20, 33, 50, 63, 64])))
         n0 = rng.randint(1, 120)
         cnt = 60
         got = o.residues(seq, param, n0, cnt, d)
