@@ -513,12 +513,16 @@ def run_serve(args, torch, ctd, nat):
                      "by_work": {"largest_10pct": pct(lat_np[large]), "other_90pct": pct(lat_np[~large])},
                      "updates_per_s": st.updates / el, "updates": st.updates,
                      "tiles": st.tiles, "active_max": st.active_max, "pending_max": st.pending_max,
+                     "scheduler_cpu_core_frac": st.scheduler_cpu_ms / 1e3 / el,
+                     "scheduler_passes": st.scheduler_passes,
+                     "device_bytes_peak": st.device_bytes_peak, "pinned_bytes_peak": st.pinned_bytes_peak,
                      "digests": dig})
         bw = runs[-1]["by_work"]
         print(f"# load {L}: {el:.1f} s, p50 {runs[-1]['p50_ms']:.1f} ms, p99 {runs[-1]['p99_ms']:.1f} ms "
               f"(largest 10 %: {bw['largest_10pct']['p50_ms']:.1f} / {bw['largest_10pct']['p99_ms']:.1f}, "
               f"others: {bw['other_90pct']['p50_ms']:.1f} / {bw['other_90pct']['p99_ms']:.1f}), "
-              f"{st.updates / el:.3e} updates/s, active max {st.active_max}", file=sys.stderr, flush=True)
+              f"{st.updates / el:.3e} updates/s, active max {st.active_max}, scheduler "
+              f"{st.scheduler_cpu_ms / 1e3 / el:.2f} of a core", file=sys.stderr, flush=True)
 
     # plain form, one job per call: the reference the served buffers are compared with
     t0 = time.perf_counter()

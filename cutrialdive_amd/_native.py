@@ -118,6 +118,10 @@ class CtdServerStats(ctypes.Structure):
         ("pending_max", u32),
         ("pending", u32),
         ("reserved", u32),
+        ("device_bytes_peak", u64),
+        ("pinned_bytes_peak", u64),
+        ("scheduler_cpu_ms", ctypes.c_double),
+        ("scheduler_passes", u64),
     ]
 
 

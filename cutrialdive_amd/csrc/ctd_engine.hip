@@ -19,6 +19,9 @@
 #include <mutex>
 #include <thread>
 #include <unordered_map>
+#include <unordered_set>
+#include <atomic>
+#include <ctime>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -761,6 +764,8 @@ int ctd_hip_server_open(ctd_hip_engine *E, const ctd_server_opts *opts, ctd_hip_
     if (S->opts.depth) S->stream_work = std::min<uint32_t>(S->opts.depth, 16) * SERVE_TILE_WORK;
     S->opts.streams = std::min<uint32_t>(S->opts.streams, 16);
     if (!S->opts.queue_limit) S->opts.queue_limit = SERVE_QUEUE_DEFAULT;
+    S->pool.limit = S->opts.memory_budget;
+    S->pinned.limit = S->opts.memory_budget;
     try {
         HIPCHK(hipSetDevice(E->device));
         if (!E->d_err) HIPCHK(hipMalloc((void **)&E->d_err, sizeof(unsigned long long)));
@@ -808,7 +813,7 @@ int ctd_hip_server_submit(ctd_hip_server *srv, const ctd_job *job, uint32_t *cou
     S->stats.submitted++;
     S->stats.pending_max = std::max<uint32_t>(S->stats.pending_max, (uint32_t)S->pending.size());
     *ticket = J->ticket;
-    S->cv_work.notify_one();
+    wake_scheduler(S);
     return CTD_OK;
 }
 
@@ -854,9 +859,14 @@ int ctd_hip_server_wait(ctd_hip_server *srv, uint64_t ticket, double *latency_ms
 int ctd_hip_server_stats(ctd_hip_server *srv, ctd_server_stats *out)
 {
     if (!srv || !out) return CTD_E_INVALID;
-    std::lock_guard<std::mutex> lk(srv->impl.mu);
-    *out = srv->impl.stats;
-    out->pending = (uint32_t)srv->impl.pending.size();
+    ctd_hip_server_impl *S = &srv->impl;
+    std::lock_guard<std::mutex> lk(S->mu);
+    *out = S->stats;
+    out->pending = (uint32_t)S->pending.size();
+    out->pinned_bytes_peak = S->pinned.peak;
+    out->scheduler_cpu_ms = S->sched_cpu_ms.load(std::memory_order_relaxed);
+    out->scheduler_passes = S->passes.load(std::memory_order_relaxed);
+    out->device_bytes_peak = S->dev_peak.load(std::memory_order_relaxed);
     return CTD_OK;
 }
 
@@ -867,8 +877,8 @@ int ctd_hip_server_close(ctd_hip_server *srv)
     {
         std::lock_guard<std::mutex> lk(S->mu);
         S->stop = true;
-        S->cv_work.notify_all();
     }
+    wake_scheduler(S);
     if (S->th.joinable()) S->th.join();
     {
         // results nobody collected still reach the caller's buffers

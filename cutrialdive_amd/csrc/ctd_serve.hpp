@@ -6,9 +6,11 @@
 // factors buffers and come back through poll/wait.
 //
 // Scheduler loop (one host thread, no device-side waits):
-//   admit   pending jobs while their factors buffers fit in the device-memory
-//           budget (the first job is always admitted) and the streams need
-//           work: allocate the device buffers, copy the host buffers in, run
+//   admit   pending jobs while their buffers fit in the memory budget and the
+//           streams need work.  A job is charged the power-of-two size classes
+//           of its device factors buffer and, separately, of its pinned
+//           staging buffer; an idle server always admits the next job.  Then:
+//           allocate the device buffers, copy the host buffers in, run
 //           the mod-2 kernel, and cut the prime window into tiles of about
 //           2^35 VALU lane-instructions (at most tile_length integers);
 //   issue   each stream keeps up to SERVE_STREAM_WORK of estimated work in
@@ -24,9 +26,11 @@
 //           completes when that copy has, which frees its device memory.
 // Results travel device -> pinned staging buffer (asynchronous) -> the
 // caller's buffers.  The last copy is done by the thread that calls
-// ctd_hip_server_poll / ctd_hip_server_wait (or close), never by the
-// scheduler: a copy into pageable memory would block the scheduler thread
-// until the DMA finishes, and no tile could be issued meanwhile.  Latency is
+// ctd_hip_server_poll / ctd_hip_server_wait (or close): a copy into pageable
+// memory would block the scheduler thread until the DMA finishes, and no tile
+// could be issued meanwhile.  Only when the staging buffers of uncollected
+// results exceed the budget does the scheduler copy a result out itself, so
+// pinned memory stays bounded whatever the caller does.  Latency is
 // measured from submit to the moment the results are back in host memory (the
 // staging copy has completed), whenever the caller collects them.
 // Residues, hits and factor buffers are those of the plain form: the same
@@ -41,18 +45,25 @@ constexpr uint32_t SERVE_DEPTH = 8; // at most this many tiles in flight per str
 constexpr uint32_t SERVE_STREAMS_DEFAULT = 3;
 constexpr double SERVE_STREAM_WORK = 2.0 * SERVE_TILE_WORK; // estimated work in flight per stream
 
-// Device buffers of finished jobs are kept for later ones, by power-of-two
-// size class: hipFree can wait for the whole device, which would serialize
-// the streams at every job completion.
-struct DevicePool {
+// Buffers of finished jobs are kept for later ones, by power-of-two size
+// class: hipFree can wait for the whole device, which would serialize the
+// streams at every job completion (and hipHostMalloc is slow and
+// synchronizing).  held counts every byte the pool has allocated and not yet
+// freed, in use or cached; before a new allocation would take held past the
+// limit, cached buffers of other classes are freed, largest first.
+template <bool PINNED>
+struct BufferPool {
     std::multimap<uint64_t, void *> free_bufs;
-    std::vector<void *> all;
+    std::unordered_set<void *> live;
+    uint64_t limit = UINT64_MAX;
+    uint64_t held = 0, cached = 0, peak = 0;
     static uint64_t size_class(uint64_t bytes)
     {
         uint64_t c = 4096;
         while (c < bytes) c <<= 1;
         return c;
     }
+    uint64_t in_use() const { return held - cached; }
     void *get(uint64_t bytes)
     {
         const uint64_t c = size_class(bytes);
@@ -60,55 +71,56 @@ struct DevicePool {
         if (it != free_bufs.end()) {
             void *p = it->second;
             free_bufs.erase(it);
+            cached -= c;
             return p;
         }
+        trim(c);
         void *p = nullptr;
-        HIPCHK(hipMalloc(&p, c));
-        all.push_back(p);
+        if constexpr (PINNED) HIPCHK(hipHostMalloc(&p, c, hipHostMallocDefault));
+        else HIPCHK(hipMalloc(&p, c));
+        live.insert(p);
+        held += c;
+        peak = std::max(peak, held);
         return p;
     }
-    void put(void *p, uint64_t bytes) { free_bufs.emplace(size_class(bytes), p); }
-    void release()
+    void put(void *p, uint64_t bytes)
     {
-        for (void *p : all) (void)hipFree(p);
-        all.clear();
-        free_bufs.clear();
+        const uint64_t c = size_class(bytes);
+        free_bufs.emplace(c, p);
+        cached += c;
     }
-};
-
-// Pinned host staging buffers by power-of-two size class, kept for later
-// jobs (hipHostMalloc is slow and synchronizing).  Shared by the scheduler
-// and the delivering threads: used under the server mutex.
-struct PinnedPool {
-    std::multimap<uint64_t, void *> free_bufs;
-    std::vector<void *> all;
-    void *get(uint64_t bytes)
+    // Frees cached buffers until `need` more bytes fit under the limit (or none is left).
+    void trim(uint64_t need)
     {
-        const uint64_t c = DevicePool::size_class(bytes);
-        auto it = free_bufs.find(c);
-        if (it != free_bufs.end()) {
-            void *p = it->second;
+        while (held + need > limit && !free_bufs.empty()) {
+            auto it = std::prev(free_bufs.end());
+            if constexpr (PINNED) HIPCHK(hipHostFree(it->second));
+            else HIPCHK(hipFree(it->second));
+            live.erase(it->second);
+            held -= it->first;
+            cached -= it->first;
             free_bufs.erase(it);
-            return p;
         }
-        void *p = nullptr;
-        HIPCHK(hipHostMalloc(&p, c, hipHostMallocDefault));
-        all.push_back(p);
-        return p;
     }
-    void put(void *p, uint64_t bytes) { free_bufs.emplace(DevicePool::size_class(bytes), p); }
     void release()
     {
-        for (void *p : all) (void)hipHostFree(p);
-        all.clear();
+        for (void *p : live) {
+            if constexpr (PINNED) (void)hipHostFree(p);
+            else (void)hipFree(p);
+        }
+        live.clear();
         free_bufs.clear();
+        held = cached = 0;
     }
 };
+using DevicePool = BufferPool<false>;
+using PinnedPool = BufferPool<true>; // shared by the scheduler and the delivering threads: used under the server mutex
 
 // Staging layout of one job: the two accumulators, then counts, then factors.
 inline uint64_t stage_counts_off() { return 2 * sizeof(unsigned long long); }
 inline uint64_t stage_factors_off(uint64_t N) { return (stage_counts_off() + N * sizeof(uint32_t) + 7) & ~7ull; }
 inline uint64_t stage_bytes(uint64_t N, uint32_t maxf) { return stage_factors_off(N) + N * (uint64_t)maxf * 8; }
+uint64_t job_pinned_bytes(uint64_t N, uint32_t maxf) { return PinnedPool::size_class(stage_bytes(N, maxf)); }
 
 struct ServeJob {
     uint64_t ticket = 0;
@@ -157,7 +169,13 @@ struct ctd_hip_server_impl {
     PinnedPool pinned; // under mu
 
     std::mutex mu;
-    std::condition_variable cv_work, cv_done;
+    std::condition_variable cv_done;
+    // Wake-ups of the scheduler: submit, close, a freed staging buffer, and a
+    // host function the device runs after each tile and each result copy.
+    // This mutex is never held across a HIP call (host functions take it).
+    std::mutex wmu;
+    std::condition_variable wcv;
+    bool wake = false;
     std::deque<ServeJob *> pending;
     std::unordered_map<uint64_t, Ticket> tickets;
     uint64_t next_ticket = 1;
@@ -167,9 +185,31 @@ struct ctd_hip_server_impl {
     ctd_server_stats stats{};
 
     std::deque<ServeJob *> active; // scheduler thread only
-    uint64_t used_bytes = 0;
+    uint64_t used_dev = 0;         // device bytes charged to admitted jobs (size classes)
+    std::atomic<double> sched_cpu_ms{0.0};
+    std::atomic<uint64_t> passes{0};
+    std::atomic<uint64_t> dev_peak{0}; // pool.peak, readable by ctd_hip_server_stats
     std::thread th;
 };
+
+void wake_scheduler(ctd_hip_server_impl *S)
+{
+    std::lock_guard<std::mutex> lk(S->wmu);
+    S->wake = true;
+    S->wcv.notify_one();
+}
+
+// Host function enqueued behind device work: the work before it has completed.
+void wake_from_device(void *user) { wake_scheduler(static_cast<ctd_hip_server_impl *>(user)); }
+
+// Pool bytes one job holds: its device factors buffer and accumulators, and
+// its pinned staging buffer (each rounded to its size class).
+uint64_t job_device_bytes(uint64_t N, uint32_t maxf)
+{
+    return DevicePool::size_class(N * sizeof(uint32_t)) + DevicePool::size_class(N * (uint64_t)maxf * 8) +
+           DevicePool::size_class(2 * sizeof(unsigned long long));
+}
+uint64_t job_pinned_bytes(uint64_t N, uint32_t maxf);
 
 hipEvent_t pool_event(ctd_hip_server_impl *S)
 {
@@ -195,13 +235,15 @@ void finish_ticket(ctd_hip_server_impl *S, ServeJob *J, int state)
     S->cv_done.notify_all();
 }
 
+void deliver(ctd_hip_server_impl *S, Ticket &t, std::unique_lock<std::mutex> &lk);
+
 // The job's results are in its pinned staging buffer (scheduler thread):
 // count it and hand it to the delivering side.
 void stage_ticket(ctd_hip_server_impl *S, ServeJob *J)
 {
     const unsigned long long *acc = (const unsigned long long *)J->stage;
     const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - J->t_submit).count();
-    std::lock_guard<std::mutex> lk(S->mu);
+    std::unique_lock<std::mutex> lk(S->mu);
     Ticket &t = S->tickets[J->ticket];
     t.state = 3;
     t.staged = J;
@@ -210,6 +252,9 @@ void stage_ticket(ctd_hip_server_impl *S, ServeJob *J)
     S->stats.primes += acc[0];
     S->stats.updates += acc[0] * (J->job.n1 > J->job.n0 ? J->job.n1 - J->job.n0 : 0);
     S->cv_done.notify_all();
+    // results nobody has collected keep their staging buffers; past the
+    // memory budget the scheduler copies them into the caller's buffers itself
+    if (S->pinned.in_use() > S->opts.memory_budget) deliver(S, t, lk);
 }
 
 // Copies a staged job's results into the caller's buffers (the calling
@@ -230,6 +275,7 @@ void deliver(ctd_hip_server_impl *S, Ticket &t, std::unique_lock<std::mutex> &lk
     t.state = 2;
     delete J;
     S->cv_done.notify_all();
+    wake_scheduler(S); // admission may be waiting for staging memory
 }
 
 // Admission of one job (scheduler thread).  Returns false when the job has to
@@ -246,12 +292,20 @@ bool admit(ctd_hip_server_impl *S, ServeJob *J)
     ctd_hip_engine *E = S->E;
     const ctd_job &q = J->job;
     const uint64_t N = q.n1 > q.n0 ? q.n1 - q.n0 : 0;
-    J->bytes = N * (4 + 8ull * q.max_factors);
-    if (!S->active.empty() && S->used_bytes + J->bytes > S->opts.memory_budget) return false;
     if (!N || q.f1 <= q.f0) { // trial_factoring_impl.hpp:380-387: nothing to do
         finish_ticket(S, J, 2);
         delete J;
         return true;
+    }
+    // the job's device and pinned pool bytes must fit in the budget next to
+    // those of the admitted jobs (and of results not yet collected); a job
+    // that arrives at an idle server is always admitted
+    J->bytes = job_device_bytes(N, q.max_factors);
+    const uint64_t pin = job_pinned_bytes(N, q.max_factors);
+    if (!S->active.empty()) {
+        if (S->used_dev + J->bytes > S->opts.memory_budget) return false;
+        std::lock_guard<std::mutex> lk(S->mu);
+        if (S->pinned.in_use() + pin > S->opts.memory_budget) return false;
     }
     const uint64_t f0 = std::max<uint64_t>(q.f0, 3);
     if (f0 < q.f1 && isqrt_u64(q.f1 - 1) > E->base_lim && E->d_sp) {
@@ -261,11 +315,12 @@ bool admit(ctd_hip_server_impl *S, ServeJob *J)
     J->d_counts = (uint32_t *)S->pool.get(N * sizeof(uint32_t));
     J->d_factors = (uint64_t *)S->pool.get(N * (uint64_t)q.max_factors * sizeof(uint64_t));
     J->d_acc = (unsigned long long *)S->pool.get(2 * sizeof(unsigned long long));
+    S->dev_peak.store(S->pool.peak, std::memory_order_relaxed);
     {
         std::lock_guard<std::mutex> lk(S->mu);
         J->stage = (unsigned char *)S->pinned.get(stage_bytes(N, q.max_factors));
     }
-    S->used_bytes += J->bytes;
+    S->used_dev += J->bytes;
     hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, S->copy_st, J->d_acc, 2u);
     if (S->opts.flags & CTD_SERVE_FRESH_BUFFERS) {
         // the job starts from an empty factors buffer: clear it on the device
@@ -401,6 +456,7 @@ bool schedule_once(ctd_hip_server_impl *S)
             issue_tile(E, J->plan, sl, T.blo, T.bhi, T.start, J->d_acc, true, SERVE_LANES_PER_CU);
             hipEvent_t ev = pool_event(S);
             HIPCHK(hipEventRecord(ev, sl.st));
+            HIPCHK(hipLaunchHostFunc(sl.st, wake_from_device, S));
             const double work = tile_work(J->plan, T.blo, T.bhi);
             J->work_left = J->tiles.empty() ? 0.0 : J->work_left - work;
             S->inflight[s].push_back(InFlight{ev, J, work});
@@ -432,6 +488,7 @@ bool schedule_once(ctd_hip_server_impl *S)
                                   S->copy_st));
             J->done = pool_event(S);
             HIPCHK(hipEventRecord(J->done, S->copy_st));
+            HIPCHK(hipLaunchHostFunc(S->copy_st, wake_from_device, S));
             J->copying = true;
             progress = true;
         }
@@ -444,7 +501,7 @@ bool schedule_once(ctd_hip_server_impl *S)
         S->pool.put(J->d_counts, N * sizeof(uint32_t));
         S->pool.put(J->d_factors, N * (uint64_t)J->job.max_factors * sizeof(uint64_t));
         S->pool.put(J->d_acc, 2 * sizeof(unsigned long long));
-        S->used_bytes -= J->bytes;
+        S->used_dev -= J->bytes;
         S->event_pool.push_back(J->ready);
         S->event_pool.push_back(J->done);
         it = S->active.erase(it);
@@ -476,21 +533,40 @@ void fail_all(ctd_hip_server_impl *S, int code, const std::string &msg)
     S->cv_done.notify_all();
 }
 
+double thread_cpu_ms()
+{
+    timespec ts{};
+    clock_gettime(CLOCK_THREAD_CPUTIME_ID, &ts);
+    return ts.tv_sec * 1e3 + ts.tv_nsec * 1e-6;
+}
+
+// The scheduler thread.  After a pass that changed nothing it sleeps until
+// woken: by a submit, by close, by a delivered result (staging memory freed),
+// or by the device itself through the host function behind every tile and
+// every result copy.  It does not poll: while jobs are on the device a 2 ms
+// time-out only bounds the wait should a wake-up ever be missed.
 void serve_loop(ctd_hip_server_impl *S)
 {
     try {
         HIPCHK(hipSetDevice(S->E->device));
         for (;;) {
-            bool progress = schedule_once(S);
-            std::unique_lock<std::mutex> lk(S->mu);
-            if (S->stop && S->pending.empty() && S->active.empty()) break;
-            if (progress) continue;
-            if (S->active.empty() && S->pending.empty()) {
-                S->cv_work.wait(lk, [&] { return S->stop || !S->pending.empty(); });
-            } else {
-                // device work in flight: poll its events again shortly
-                S->cv_work.wait_for(lk, std::chrono::microseconds(10));
+            const bool progress = schedule_once(S);
+            S->passes.fetch_add(1, std::memory_order_relaxed);
+            S->sched_cpu_ms.store(thread_cpu_ms(), std::memory_order_relaxed);
+            {
+                std::lock_guard<std::mutex> lk(S->mu);
+                if (S->stop && S->pending.empty() && S->active.empty()) break;
             }
+            if (progress) continue;
+            std::unique_lock<std::mutex> wl(S->wmu);
+            if (S->active.empty()) {
+                S->wcv.wait(wl, [&] { return S->wake; });
+            } else {
+                // work is on the device: its host functions wake the thread;
+                // the time-out only bounds the wait should one ever be missed
+                S->wcv.wait_for(wl, std::chrono::milliseconds(2), [&] { return S->wake; });
+            }
+            S->wake = false;
         }
     } catch (const HipError &h) {
         fail_all(S, h.code == hipErrorOutOfMemory ? CTD_E_NOMEM : CTD_E_DEVICE,

@@ -19,7 +19,7 @@
 extern "C" {
 #endif
 
-#define CTD_HIP_ABI_VERSION 1
+#define CTD_HIP_ABI_VERSION 2
 
 enum {
     CTD_OK = 0,
@@ -199,7 +199,10 @@ int ctd_hip_batch_shape(ctd_hip_engine *eng, const ctd_job *job, uint32_t *block
 typedef struct ctd_hip_server ctd_hip_server;
 
 typedef struct ctd_server_opts {
-    uint64_t memory_budget; /* device bytes for admitted jobs' factors buffers (0 = 8 GiB) */
+    uint64_t memory_budget; /* bytes of job buffers, counted in power-of-two size classes:
+                               device factors buffers of admitted jobs, and separately the
+                               pinned staging buffers of admitted and uncollected jobs
+                               (0 = 8 GiB each); cached buffers are freed to stay under it */
     uint64_t tile_length;   /* integers of a prime window per tile (0 = 2^26) */
     uint32_t streams;       /* streams = resident prime slots (0 = 3, at most 16) */
     uint32_t queue_limit;   /* jobs waiting for admission before submit returns CTD_E_BUSY (0 = 65536) */
@@ -222,6 +225,10 @@ typedef struct ctd_server_stats {
     uint32_t pending_max;   /* largest queue of jobs waiting for admission */
     uint32_t pending;       /* jobs waiting for admission now */
     uint32_t reserved;
+    uint64_t device_bytes_peak; /* most device memory the job-buffer pool held (in use + cached) */
+    uint64_t pinned_bytes_peak; /* most pinned host memory the staging pool held */
+    double scheduler_cpu_ms;    /* CPU time of the scheduler thread so far */
+    uint64_t scheduler_passes;  /* scheduler passes so far (each follows a wake-up or progress) */
 } ctd_server_stats;
 
 int ctd_hip_server_open(ctd_hip_engine *eng, const ctd_server_opts *opts, ctd_hip_server **out);

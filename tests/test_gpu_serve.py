@@ -177,3 +177,108 @@ def test_fifo_policy_gives_the_same_buffers(engine):
     for s, (c, f) in list(zip(jobs, out[False]))[:8]:
         rc, rf = oracle.trial_factor_raw(s.seq_id, s.param, s.n0, s.n1, s.f0, s.f1, 64)
         assert_same(c, f, rc, rf, 64, s)
+
+
+def size_class(b):
+    c = 4096
+    while c < b:
+        c <<= 1
+    return c
+
+
+def device_charge(N, mx):
+    # the job's device factors buffer and accumulators, in pool size classes
+    return size_class(4 * N) + size_class(8 * N * mx) + size_class(16)
+
+
+def pinned_charge(N, mx):
+    # its pinned staging buffer: accumulators, counts, factors
+    return size_class(((16 + 4 * N + 7) & ~7) + 8 * N * mx)
+
+
+def test_config5_poisson_stream(engine):
+    """Config 5 at test size: a seeded Poisson stream of jobs drawn from the
+    mixed-job distribution (7 families, 10^2..10^5 indices, windows in
+    [2^38, 2^43)) with each window cut to 2^17 integers so that the oracle
+    covers every job, offered at 0.9 of the server's capacity measured on the
+    same jobs.  Every served job equals the oracle and the same job run alone."""
+    import os
+    import threading
+
+    specs = workloads.mixed_jobs(120, seed=workloads.SEED + 5, window_bits=17)
+    jobs = [job_of(s) for s in specs]
+    threads = min(os.cpu_count() or 1, 16)
+    print(f"[cfg5] oracle on {len(specs)} jobs, {threads} threads", flush=True)
+    ref = [oracle.trial_factor_raw(s.seq_id, s.param, s.n0, s.n1, s.f0, s.f1, s.max_factors, threads) for s in specs]
+    with ctd.Server(engine, fresh_buffers=True) as srv:
+        print("[cfg5] each job alone", flush=True)
+        alone = [srv.wait(srv.submit(j))[:2] for j in jobs]
+        print("[cfg5] capacity: every job at once", flush=True)
+        t0 = time.perf_counter()
+        tickets = [srv.submit(j) for j in jobs]
+        for t in tickets:
+            srv.wait(t)
+        capacity = len(jobs) / (time.perf_counter() - t0)
+        arrivals = workloads.poisson_arrivals(len(jobs), 0.9 * capacity, seed=workloads.SEED + 6)
+        print(f"[cfg5] Poisson stream at 0.9 x {capacity:.0f} jobs/s", flush=True)
+        out = [None] * len(jobs)
+        ts = [None] * len(jobs)
+        ready = threading.Semaphore(0)
+
+        def consume():
+            for k in range(len(jobs)):
+                ready.acquire()
+                out[k] = srv.wait(ts[k])
+
+        th = threading.Thread(target=consume)
+        th.start()
+        t_start = time.perf_counter()
+        for k, j in enumerate(jobs):
+            dt = t_start + arrivals[k] - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            ts[k] = srv.submit(j)
+            ready.release()
+        th.join()
+        st = srv.stats()
+    assert st.failed == 0 and st.completed == 3 * len(jobs)
+    assert st.active_max > 1  # jobs overlapped on the device
+    for s, (c, f, lat), (ac, af), (rc, rf) in zip(specs, out, alone, ref):
+        assert lat > 0
+        assert_same(c, f, rc, rf, s.max_factors, (s.kind, s.param, s.n0, s.n1, s.f0, "vs oracle"))
+        assert_same(c, f, ac, af, s.max_factors, (s.kind, s.param, s.n0, s.n1, s.f0, "vs alone"))
+
+
+def test_pool_bytes_stay_within_the_budget(engine):
+    """Admission charges each job the size classes of its device and pinned
+    buffers, cached buffers are freed to stay under the budget, and results
+    nobody collects do not pile up in pinned memory: the pools' peaks stay
+    within the budget plus one job (an idle server always admits a job)."""
+    budget = 8 << 20
+    rng = np.random.default_rng(11)
+    specs = []
+    for k in range(60):
+        N = int(rng.choice([100, 900, 5000, 30000, 120000]))
+        mx = int(rng.choice([4, 16, 32]))
+        n0 = int(rng.integers(2, 500))
+        f0 = int(rng.choice([3, 1 << 30, 1 << 40]))
+        specs.append(workloads.JobSpec("x", 1, 3 + 2 * k, n0, n0 + N, f0, f0 + (1 << 16), mx))
+    with ctd.Server(engine, memory_budget=budget, fresh_buffers=True, tile_length=1 << 20) as srv:
+        tickets = [srv.submit(job_of(s)) for s in specs]
+        deadline = time.time() + 120
+        while srv.stats().completed < len(specs):  # nothing collected until every job is done
+            assert time.time() < deadline
+            time.sleep(0.01)
+        res = [srv.wait(t)[:2] for t in tickets]
+        st = srv.stats()
+    dev1 = max(device_charge(s.n1 - s.n0, s.max_factors) for s in specs)
+    pin1 = max(pinned_charge(s.n1 - s.n0, s.max_factors) for s in specs)
+    print(f"[budget] device peak {st.device_bytes_peak}, pinned peak {st.pinned_bytes_peak}, "
+          f"one job at most {dev1} / {pin1}, scheduler {st.scheduler_cpu_ms:.1f} ms CPU in "
+          f"{st.scheduler_passes} passes", flush=True)
+    assert 0 < st.device_bytes_peak <= max(budget, dev1)
+    assert 0 < st.pinned_bytes_peak <= budget + pin1
+    assert st.scheduler_passes > 0
+    for s, (c, f) in list(zip(specs, res))[::6]:
+        rc, rf = oracle.trial_factor_raw(s.seq_id, s.param, s.n0, s.n1, s.f0, s.f1, s.max_factors)
+        assert_same(c, f, rc, rf, s.max_factors, s)
