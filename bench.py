@@ -177,6 +177,48 @@ def probe_peak(eng):
     return max(best(op) for op in LOOP_PROBE_OPS), best(0)
 
 
+# Algorithmic cost contract, fixed before the first kernel was written:
+# full-rate int32 lane-ops per update of each family's plain recurrence in the
+# reference's form (one Barrett multiply-reduce or modular add per index), for
+# primes below 2^31 (W32) and above (W64).  Independent of how this engine's
+# loops compute: a form that does less work per update than the recurrence
+# scores above 1 against it.
+C_CONTRACT = {
+    "mersenne": (4, 6), "proth": (4, 6), "riesel": (4, 6),
+    "proth_fixed_n": (4, 7), "riesel_fixed_n": (4, 7),
+    "fibonacci": (4, 7), "lucas": (4, 7), "tribonacci": (7, 13),
+}
+NOMINAL_VALU = 39.3e12  # 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz, lane-ops/s
+
+
+def contract_ops(seq_id: int, param: int, pmax: int) -> float:
+    fam = FAMILY_NAMES.get(seq_id, "peano")
+    if fam == "smarandache":  # power-of-2 bases shift instead of multiplying
+        return 32.0 if param & (param - 1) == 0 else 40.0
+    w32, w64 = C_CONTRACT.get(fam, (4, 6))
+    return float(w32 if pmax < (1 << 31) else w64)
+
+
+def cpu_share():
+    """(threads, description): the CPUs this process may use -- its affinity
+    set, capped by a cgroup CPU quota and by OMP_NUM_THREADS when the machine
+    sets them to the job's share."""
+    n = len(os.sched_getaffinity(0))
+    src = [f"affinity {n}"]
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            n = min(n, max(1, int(q) // int(per)))
+            src.append(f"cgroup quota {int(q) / int(per):g}")
+    except (OSError, ValueError):
+        pass
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+        src.append(f"OMP_NUM_THREADS {omp}")
+    return n, ", ".join(src) + f"; host logical CPUs {os.cpu_count()}"
+
+
 def cpu_sample(jobs, sample_len, threads):
     """Oracle (the C restatement of the reference CPU path) on the first
     sample_len integers of every job's window."""
@@ -202,6 +244,166 @@ def same_buffers(c, f, rc, rf, mx):
     return True
 
 
+class DeviceBuffers:
+    """The factors buffers of a list of jobs, resident in HBM (one torch
+    allocation each for counts and factors)."""
+
+    def __init__(self, torch, jobs, mx):
+        self.Ns = [j.n1 - j.n0 for j in jobs]
+        self.mx = mx
+        self.offs = np.concatenate([[0], np.cumsum(self.Ns)]).astype(np.int64)
+        self.counts = torch.zeros(int(self.offs[-1]), dtype=torch.int32, device="cuda")
+        self.factors = torch.zeros(int(self.offs[-1]) * mx, dtype=torch.int64, device="cuda")
+        self.cptr = [self.counts.data_ptr() + 4 * int(self.offs[k]) for k in range(len(jobs))]
+        self.fptr = [self.factors.data_ptr() + 8 * mx * int(self.offs[k]) for k in range(len(jobs))]
+
+    def zero(self):
+        self.counts.zero_()
+        self.factors.zero_()
+
+    def host(self):
+        return self.counts.cpu().numpy().view(np.uint32), self.factors.cpu().numpy().view(np.uint64)
+
+    def job(self, k, c, f):
+        o = self.offs
+        return c[o[k]:o[k + 1]], f[o[k] * self.mx:o[k + 1] * self.mx]
+
+
+def batched_vs_plain(torch, eng, nat, cjobs, mx, flags=0, tile_length=0, streams=0, steps=1, plain=True,
+                     label="", bufs=None):
+    """Times the batched form (one ctd_hip_run_batch per step) and then the
+    plain form (one ctd_hip_trial_factor_device per job) on the same jobs and
+    HBM-resident buffers, and compares the two sets of buffers."""
+    bufs = bufs or DeviceBuffers(torch, cjobs, mx)
+    out = {"jobs": len(cjobs)}
+    upd = primes = graphs = tiles = 0
+    bms = build = 0.0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        if k:
+            bufs.zero()
+        st = eng.run_batch_raw(cjobs, bufs.cptr, bufs.fptr, flags, tile_length, streams)
+        upd += st.updates
+        primes += st.primes
+        bms += st.total_ms
+        build += st.build_ms
+        graphs += st.graphs
+        tiles += st.tiles
+        print(f"# {label} batched step {k}: {st.tiles} tiles, {st.graphs} graph(s), graph {st.total_ms:.0f} ms, "
+              f"build {st.build_ms:.0f} ms", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    out.update(value=upd / dt, seconds=dt, updates=upd, primes=primes, graph_ms=bms, build_ms=build, graphs=graphs,
+               tiles=tiles)
+    if not plain:
+        return out, bufs, None
+    bc, bf = bufs.host()
+    bufs.zero()
+    torch.cuda.synchronize()
+    p_upd = p_primes = 0
+    sieve_ms = tf_ms = 0.0
+    per_job = []
+    t0 = time.perf_counter()
+    for k, c in enumerate(cjobs):
+        eng.run_job_device(c, bufs.cptr[k], bufs.fptr[k])
+        st = eng.last_stats()
+        p_upd += st.updates
+        p_primes += st.primes
+        sieve_ms += st.sieve_ms
+        tf_ms += st.tf_ms
+        per_job.append(st.updates)
+    torch.cuda.synchronize()
+    pdt = time.perf_counter() - t0
+    pc, pf = bufs.host()
+    match = all(same_buffers(*bufs.job(k, bc, bf), *bufs.job(k, pc, pf), mx) for k in range(len(cjobs)))
+    print(f"# {label} plain form {pdt:.1f} s, identical factors: {match}", file=sys.stderr, flush=True)
+    plain_res = {"value": p_upd / pdt, "seconds": pdt, "updates": p_upd, "launch_form": "one job per call",
+                 "factors_identical_to_batched": bool(match), "sieve_ms": sieve_ms, "tf_ms": tf_ms,
+                 "primes": p_primes}
+    out["batched_vs_plain"] = out["value"] / plain_res["value"]
+    return out, bufs, (plain_res, per_job)
+
+
+def serve_block(args, eng, ctd, nat, jobs, cjobs, batched_digests):
+    """The serving form on the mixed-job distribution (config 5, bounded):
+    capacity = the 256 mixed jobs submitted at once; then a seeded Poisson
+    stream of jobs drawn from the same distribution (2^30-wide windows in
+    [2^38, 2^43)) at args.serve_load of that capacity.  Every served job's
+    buffers are compared with the batched form's on the same jobs."""
+    import threading
+
+    from cutrialdive_amd import workloads
+
+    mx = jobs[0].max_factors
+
+    def serve(sub, arrivals):
+        srv = ctd.Server(eng, fresh_buffers=True)
+        lat, dig, tickets = [0.0] * len(sub), [None] * len(sub), [None] * len(sub)
+        ready = threading.Semaphore(0)
+
+        def consume():
+            for k in range(len(sub)):
+                ready.acquire()
+                c, f, ms = srv.wait(tickets[k])
+                lat[k] = ms
+                dig[k] = buffer_digest(c, f, mx)
+
+        th = threading.Thread(target=consume)
+        th.start()
+        t_start = time.perf_counter()
+        for k, c in enumerate(sub):
+            if arrivals is not None:
+                dt = t_start + arrivals[k] - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+            tickets[k] = srv.submit(c)
+            ready.release()
+        th.join()
+        el = time.perf_counter() - t_start
+        st = srv.stats()
+        srv.close()
+        return lat, dig, el, st
+
+    _, cap_dig, cap_s, cap_st = serve(cjobs, None)
+    capacity = len(cjobs) / cap_s
+    print(f"# serving capacity on the {len(cjobs)} mixed jobs: {capacity:.1f} jobs/s "
+          f"({cap_st.updates / cap_s:.3e} updates/s)", file=sys.stderr, flush=True)
+    specs = workloads.mixed_jobs(args.serve_block_jobs, seed=workloads.SEED + 2)
+    sjobs = [nat.CtdJob(j.seq_id, j.max_factors, j.param, j.n0, j.n1, j.f0, j.f1) for j in specs]
+    arr = workloads.poisson_arrivals(len(sjobs), args.serve_load * capacity, seed=workloads.SEED + 3)
+    lat, dig, el, st = serve(sjobs, arr)
+    lat = np.array(lat)
+    print(f"# serving stream: {len(sjobs)} jobs at load {args.serve_load}: {el:.1f} s, p50 "
+          f"{np.percentile(lat, 50):.0f} ms, p99 {np.percentile(lat, 99):.0f} ms, {st.updates / el:.3e} updates/s",
+          file=sys.stderr, flush=True)
+    res, _ = eng.run_batch(sjobs)
+    ref = [buffer_digest(c, f, mx) for c, f in res]
+    return {
+        "workload": f"Poisson stream of {len(sjobs)} jobs of the mixed-job distribution (2^30-wide windows in "
+                    f"[2^38, 2^43), seed {workloads.SEED + 2}) at offered load {args.serve_load} of the measured "
+                    "capacity",
+        "capacity_jobs_per_s": capacity,
+        "capacity_updates_per_s": cap_st.updates / cap_s,
+        "capacity_sample": f"the {len(cjobs)} mixed jobs submitted at once",
+        "capacity_identical_to_batched": cap_dig == batched_digests,
+        "load": args.serve_load,
+        "offered_jobs_per_s": args.serve_load * capacity,
+        "jobs": len(sjobs),
+        "seconds": el,
+        "p50_ms": float(np.percentile(lat, 50)),
+        "p99_ms": float(np.percentile(lat, 99)),
+        "mean_ms": float(lat.mean()),
+        "updates_per_s": st.updates / el,
+        "updates": st.updates,
+        "active_max": st.active_max,
+        "pending_max": st.pending_max,
+        "scheduler_cpu_core_frac": st.scheduler_cpu_ms / 1e3 / el,
+        "factors_identical_to_batched": dig == ref,
+        "latency": "submit to results back in host memory",
+    }
+
+
 def run_mixed(args, torch, ctd, nat):
     from cutrialdive_amd import workloads
 
@@ -210,96 +412,63 @@ def run_mixed(args, torch, ctd, nat):
         jobs = [j for j in jobs if j.kind == args.family]
     eng = ctd.Engine(0)
     name, mem, cu = eng.device_info()
-    Ns = [j.n1 - j.n0 for j in jobs]
     mx = jobs[0].max_factors
     print(f"# device {name}, {cu} CUs, {mem / 2**30:.0f} GiB; {len(jobs)} mixed jobs, "
-          f"{sum(Ns)} indices, ~{workloads.updates_of(jobs):.3e} updates", file=sys.stderr, flush=True)
-    offs = np.concatenate([[0], np.cumsum(Ns)])
-    d_counts = torch.zeros(int(offs[-1]), dtype=torch.int32, device="cuda")
-    d_factors = torch.zeros(int(offs[-1]) * mx, dtype=torch.int64, device="cuda")
-    cptr = [d_counts.data_ptr() + 4 * int(offs[k]) for k in range(len(jobs))]
-    fptr = [d_factors.data_ptr() + 8 * mx * int(offs[k]) for k in range(len(jobs))]
+          f"{sum(j.n1 - j.n0 for j in jobs)} indices, ~{workloads.updates_of(jobs):.3e} updates", file=sys.stderr,
+          flush=True)
     cjobs = [nat.CtdJob(j.seq_id, j.max_factors, j.param, j.n0, j.n1, j.f0, j.f1) for j in jobs]
     flags = (nat.CTD_BATCH_NO_AUTOTUNE if args.no_autotune else 0) | (nat.CTD_BATCH_NO_GRAPH if args.no_graph else 0)
+    tile = args.tile_bits and 1 << args.tile_bits
+    bufs = DeviceBuffers(torch, cjobs, mx)
 
     # warm-up: launch-shape probes for every (family, kind, index class), then
     # a batch on the first 2^24 integers of every window
     t0 = time.perf_counter()
     warm = [nat.CtdJob(j.seq_id, j.max_factors, j.param, j.n0, j.n1, j.f0, j.f0 + (1 << 24)) for j in jobs]
     for _ in range(args.warmup):
-        wst = eng.run_batch_raw(warm, cptr, fptr, flags, args.tile_bits and 1 << args.tile_bits)
+        eng.run_batch_raw(warm, bufs.cptr, bufs.fptr, flags, tile)
     tune_s = time.perf_counter() - t0
     shapes = {}
     if not args.no_autotune:
         for j, c in zip(jobs, cjobs):
-            shapes.setdefault(f"{j.kind}/N{'<1k' if j.n1 - j.n0 < 1024 else '<16k' if j.n1 - j.n0 < 16384 else '>=16k'}",
-                              eng.batch_shape(c))
-    d_counts.zero_()
-    d_factors.zero_()
-    torch.cuda.synchronize()
+            key = f"{j.kind}/N{'<1k' if j.n1 - j.n0 < 1024 else '<16k' if j.n1 - j.n0 < 16384 else '>=16k'}"
+            shapes.setdefault(key, eng.batch_shape(c))
+    bufs.zero()
     print(f"# warm-up + autotune {tune_s:.1f} s", file=sys.stderr, flush=True)
 
-    # batched form (timed)
-    upd = primes = 0
-    bms = build = 0.0
-    graphs = tiles = 0
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        if k:
-            d_counts.zero_()
-            d_factors.zero_()
-        st = eng.run_batch_raw(cjobs, cptr, fptr, flags, args.tile_bits and 1 << args.tile_bits, args.batch_streams)
-        upd += st.updates
-        primes += st.primes
-        bms += st.total_ms
-        build += st.build_ms
-        graphs += st.graphs
-        tiles += st.tiles
-        print(f"# batched step {k}: {st.tiles} tiles, {st.graphs} graph(s), graph {st.total_ms:.0f} ms, "
-              f"build {st.build_ms:.0f} ms", file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    value = upd / dt
-    bat_counts = d_counts.cpu().numpy().view(np.uint32)
-    bat_factors = d_factors.cpu().numpy().view(np.uint64)
-
-    # plain form: one job per call, same buffers
+    # the timed step: the batched form; then the plain form on the same jobs
+    bat, bufs, pl = batched_vs_plain(torch, eng, nat, cjobs, mx, flags, tile, args.batch_streams, args.steps,
+                                     plain=not args.no_plain, label="mixed", bufs=bufs)
+    value, upd, primes = bat["value"], bat["updates"], bat["primes"]
+    graphs = max(bat["graphs"], 1)
     result_plain = None
-    if not args.no_plain:
-        d_counts.zero_()
-        d_factors.zero_()
-        torch.cuda.synchronize()
-        p_upd = 0
-        t0 = time.perf_counter()
-        for k, c in enumerate(cjobs):
-            eng.run_job_device(c, cptr[k], fptr[k])
-            p_upd += eng.last_stats().updates
-        torch.cuda.synchronize()
-        pdt = time.perf_counter() - t0
-        pc = d_counts.cpu().numpy().view(np.uint32)
-        pf = d_factors.cpu().numpy().view(np.uint64)
-        match = all(same_buffers(bat_counts[offs[k]:offs[k + 1]], bat_factors[offs[k] * mx:offs[k + 1] * mx],
-                                 pc[offs[k]:offs[k + 1]], pf[offs[k] * mx:offs[k + 1] * mx], mx)
-                    for k in range(len(jobs)))
-        result_plain = {"value": p_upd / pdt, "seconds": pdt, "updates": p_upd, "launch_form": "one job per call",
-                        "factors_identical_to_batched": bool(match)}
-        print(f"# plain form {pdt:.1f} s, identical factors: {match}", file=sys.stderr, flush=True)
+    sieve = None
+    c_contract = None
+    if pl:
+        result_plain, per_job = pl
+        # the plain form's sieve launches are timed on their own (HIP events)
+        sms = result_plain["sieve_ms"]
+        sieve = {"primes_per_s": result_plain["primes"] / (sms / 1e3), "sieve_ms": sms,
+                 "prime_stream_GBps": 8.0 * result_plain["primes"] / (sms / 1e3) / 1e9,
+                 "form": "plain form, sieve_kernel launches of the 256 windows (2^31-integer batches), HIP events; "
+                         "8 B per prime written to HBM and read once by the residue kernel"}
+        w = np.array(per_job, dtype=np.float64)
+        cs = np.array([contract_ops(c.seq_id, c.param, c.f1 - 1) for c in cjobs])
+        c_contract = float((w * cs).sum() / w.sum())
         # the plain form with the reference kernel's arithmetic shape: one
         # 64-bit modular multiply-reduce per update (CTD_RUN_GENERIC_ONLY),
         # one job per call, on the first 2^generic_sample_bits integers of each
         # window (a bounded sample: this form is an order of magnitude slower)
         if args.generic_sample_bits:
             opts = nat.CtdRunOpts(0, 0, 0, 0, nat.CTD_RUN_GENERIC_ONLY)
-            d_counts.zero_()
-            d_factors.zero_()
+            bufs.zero()
             torch.cuda.synchronize()
             g_upd = 0
             t0 = time.perf_counter()
             for k, c in enumerate(cjobs):
-                w = nat.CtdJob(c.seq_id, c.max_factors, c.param, c.n0, c.n1, c.f0,
-                               min(c.f1, c.f0 + (1 << args.generic_sample_bits)))
-                eng.run_job_device(w, cptr[k], fptr[k], opts=opts)
+                wj = nat.CtdJob(c.seq_id, c.max_factors, c.param, c.n0, c.n1, c.f0,
+                                min(c.f1, c.f0 + (1 << args.generic_sample_bits)))
+                eng.run_job_device(wj, bufs.cptr[k], bufs.fptr[k], opts=opts)
                 g_upd += eng.last_stats().updates
             torch.cuda.synchronize()
             gdt = time.perf_counter() - t0
@@ -311,15 +480,83 @@ def run_mixed(args, torch, ctd, nat):
             result_plain["batched_vs_generic_plain"] = value / (g_upd / gdt)
             print(f"# generic plain form on the sample: {g_upd / gdt:.3e} updates/s ({gdt:.1f} s)", file=sys.stderr,
                   flush=True)
+    batched_digests = None
+    if args.serve_block_jobs:
+        bc, bf = None, None
+        # the batched form's buffers of the timed step, for the serving comparison
+        bufs.zero()
+        eng.run_batch_raw(cjobs, bufs.cptr, bufs.fptr, flags, tile, args.batch_streams)
+        bc, bf = bufs.host()
+        batched_digests = [buffer_digest(*bufs.job(k, bc, bf), mx) for k in range(len(cjobs))]
+    del bufs
+    torch.cuda.empty_cache()
+
+    # short jobs: the same distribution on 2^24-integer windows, where one job
+    # no longer fills the device
+    short = None
+    if args.short_jobs and not args.no_plain:
+        sj = workloads.mixed_jobs(args.short_jobs, window_bits=24)
+        sc = [nat.CtdJob(j.seq_id, j.max_factors, j.param, j.n0, j.n1, j.f0, j.f1) for j in sj]
+        eng.run_batch(sc[:64])  # launch shapes of the short windows' index classes
+        sb, sbufs, (sp, _) = batched_vs_plain(torch, eng, nat, sc, mx, flags, tile, args.batch_streams, 1,
+                                              label="2^24-window mix")
+        del sbufs
+        torch.cuda.empty_cache()
+        short = {"workload": f"{len(sc)} jobs of the mixed-job distribution with 2^24-wide prime windows",
+                 "batched": {k: sb[k] for k in ("value", "seconds", "updates", "tiles", "graphs")},
+                 "plain": {k: sp[k] for k in ("value", "seconds", "updates", "factors_identical_to_batched")},
+                 "batched_vs_plain": sb["batched_vs_plain"]}
+
+    serving = serve_block(args, eng, ctd, nat, jobs, cjobs, batched_digests) if args.serve_block_jobs else None
 
     c_upd, c_src = mixed_ops(jobs, full_batch=args.family is None and args.jobs == 256)
     peak, peak_add = probe_peak(eng)
-    per_launch_s = bms / 1e3 / max(graphs, 1)
-    achieved = upd / max(graphs, 1) * c_upd / per_launch_s / 1e12
+    per_launch_s = bat["graph_ms"] / 1e3 / graphs
+    upd_launch = upd / graphs
+    achieved = upd_launch * c_upd / per_launch_s / 1e12
     sm = ops_table()["measured"]
     # the sieve's measured write and the residue kernel's measured read per prime
     traffic = (sm["sieve"]["hbm_write_bytes_per_prime"] + sm["sieve"]["hbm_read_bytes_per_prime"]
-               + sm["cfg2"]["hbm_read_bytes_per_prime"]) * primes / max(graphs, 1)
+               + sm["cfg2"]["hbm_read_bytes_per_prime"]) * primes / graphs
+    upd_rate_launch = upd_launch / per_launch_s
+    roof = {
+        "bound": "valu",
+        "achieved": achieved,
+        "peak": peak,
+        "unit": "T VALU lane-ops/s",
+        "frac": achieved / peak,
+        "frac_basis": "VALU issue: updates/s x the loops' VALU instructions per update (ops_per_update) / the "
+                      "fastest probed rate of those instructions",
+        "traffic": traffic,
+        "traffic_unit": "HBM bytes per graph launch (prime stream: sieve write + residue read)",
+        "traffic_source": "per-prime bytes measured with rocprofv3 FETCH_SIZE x 2.0 / WRITE_SIZE on the "
+                          "same kernels, profiles/tf_fast_smarandache_cfg2.md",
+        "kernel": "batch graph (sieve_kernel + tf_fast_kernel<family> per tile)",
+        "ops_per_update": c_upd,
+        "ops_per_update_source": c_src,
+        "peak_source": "fastest of the loops' instructions (v_add_f64, v_fma_f64, v_mul_f64, "
+                       "v_cmp_eq_f64), independent-chain probe, best of 3 runs in this run",
+        "v_add_u32_peak": peak_add,
+        "frac_of_v_add_u32_peak": achieved / peak_add,
+        "frac_of_v_add_u32_peak_basis": "the same achieved rate against the probed v_add_u32 rate of this run, "
+                                        "the VALU integer peak the >= 60 % target is quoted against",
+        "nominal_peak": NOMINAL_VALU / 1e12,
+        "updates_per_s_per_launch": upd_rate_launch,
+        "one_compare_frac": upd_rate_launch / NOMINAL_VALU,
+        "one_compare_basis": "updates/s / 39.3e12: every update costs at least one compare issued at the nominal "
+                             "rate",
+        "avg_launch_ms": per_launch_s * 1e3,
+        "hbm": {"achieved_GBps": traffic / per_launch_s / 1e9, "peak_GBps": HBM_PEAK_GBPS,
+                "frac": traffic / per_launch_s / 1e9 / HBM_PEAK_GBPS},
+    }
+    if c_contract is not None:
+        roof["algorithmic_ops_per_update"] = c_contract
+        roof["algorithmic_frac"] = upd_rate_launch * c_contract / NOMINAL_VALU
+        roof["algorithmic_basis"] = ("updates/s x the fixed cost contract (int32 lane-ops per update of each "
+                                     "family's plain Barrett recurrence, W64 here: doubling 6, fixed-n 7, "
+                                     "Fibonacci/Lucas 7, Tribonacci 13, Smarandache 40 / 32 for power-of-2 bases), "
+                                     "update-weighted over the jobs / 39.3e12; above 1 where a form replaces the "
+                                     "per-update recurrence by fewer operations")
     result = {
         "metric": METRIC,
         "value": value,
@@ -327,7 +564,7 @@ def run_mixed(args, torch, ctd, nat):
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": dt * 1e3 / max(args.steps, 1),
+        "ms_per_step": bat["seconds"] * 1e3 / max(args.steps, 1),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -339,42 +576,24 @@ def run_mixed(args, torch, ctd, nat):
             "workload": f"mixed batch of {len(jobs)} seeded jobs (7 families, 10^2..10^5 indices, "
                         f"2^{args.window_bits}-wide prime windows in [2^38, 2^43)), batched form, one hipGraph launch",
             "jobs": len(jobs),
-            "indices": int(sum(Ns)),
+            "indices": int(sum(j.n1 - j.n0 for j in jobs)),
             "primes_timed": primes,
             "updates_timed": upd,
-            "tiles": tiles // max(args.steps, 1),
-            "graph_launches_per_step": graphs // max(args.steps, 1),
-            "graph_build_ms_per_step": build / max(args.steps, 1),
+            "tiles": bat["tiles"] // max(args.steps, 1),
+            "graph_launches_per_step": bat["graphs"] // max(args.steps, 1),
+            "graph_build_ms_per_step": bat["build_ms"] / max(args.steps, 1),
             "autotune": not args.no_autotune,
             "shapes": {k: {"block": b, "waves_per_cu": w, "chunk_split": s} for k, (b, w, s) in sorted(shapes.items())},
         },
         "plain": result_plain,
         "batched_vs_plain": (value / result_plain["value"]) if result_plain else None,
-        "roofline": {
-            "bound": "valu",
-            "achieved": achieved,
-            "peak": peak,
-            "unit": "T VALU lane-ops/s",
-            "frac": achieved / peak,
-            "traffic": traffic,
-            "traffic_unit": "HBM bytes per graph launch (prime stream: sieve write + residue read)",
-            "traffic_source": "per-prime bytes measured with rocprofv3 FETCH_SIZE x 2.0 / WRITE_SIZE on the "
-                              "same kernels, profiles/tf_fast_smarandache_cfg2.md",
-            "kernel": "batch graph (sieve_kernel + tf_fast_kernel<family> per tile)",
-            "ops_per_update": c_upd,
-            "ops_per_update_source": c_src,
-            "peak_source": "fastest of the loops' instructions (v_add_f64, v_fma_f64, v_mul_f64, "
-                           "v_cmp_eq_f64), independent-chain probe, best of 3 runs in this run",
-            "v_add_u32_peak": peak_add,
-            "frac_of_v_add_u32_peak": achieved / peak_add,
-            "nominal_peak": 39.3,
-            "avg_launch_ms": per_launch_s * 1e3,
-            "hbm": {"achieved_GBps": traffic / per_launch_s / 1e9, "peak_GBps": HBM_PEAK_GBPS,
-                    "frac": traffic / per_launch_s / 1e9 / HBM_PEAK_GBPS},
-        },
+        "short_windows": short,
+        "sieve": sieve,
+        "serving": serving,
+        "roofline": roof,
     }
     if not args.no_cpu:
-        threads = min(os.cpu_count() or 1, 16)
+        threads, share = cpu_share()
         sample = 1 << args.cpu_sample_bits
         ref, cupd, cdt = cpu_sample(jobs, sample, threads)
         sjobs = [nat.CtdJob(j.seq_id, j.max_factors, j.param, j.n0, j.n1, j.f0, j.f0 + sample) for j in jobs]
@@ -384,6 +603,7 @@ def run_mixed(args, torch, ctd, nat):
             "value": cupd / cdt,
             "unit": "updates/s",
             "cores": threads,
+            "cores_source": share,
             "kind": "port",
             "sample": f"oracle (C restatement of the reference CPU path) on the first 2^{args.cpu_sample_bits} "
                       f"integers of each of the {len(jobs)} job windows ({cupd:.3e} updates), {cdt:.1f} s "
@@ -545,7 +765,7 @@ def run_serve(args, torch, ctd, nat):
 
         work = sorted(range(len(specs)), key=lambda k: (specs[k].n1 - specs[k].n0) * (specs[k].f1 - specs[k].f0))
         pick = work[:args.serve_cpu_jobs]
-        threads = min(os.cpu_count() or 1, 16)
+        threads, _ = cpu_share()
         t0 = time.perf_counter()
         ok = True
         for k in pick:
@@ -733,7 +953,7 @@ def run_single(args, torch, ctd, nat):
     if not args.no_cpu:
         from oracle import oracle as o
 
-        threads = min(os.cpu_count() or 1, 16)
+        threads, share = cpu_share()
         sample = 1 << args.cpu_sample_bits
         f0s = F0 if F0 > 2 else 3
         t0 = time.perf_counter()
@@ -746,6 +966,7 @@ def run_single(args, torch, ctd, nat):
             "value": cpr * N / cdt,
             "unit": "updates/s",
             "cores": threads,
+            "cores_source": share,
             "kind": "port",
             "sample": f"oracle (C restatement of the reference CPU path) on primes in [{f0s}, {f0s + sample}) "
                       f"({cpr} primes x {N} indices), {cdt:.1f} s with {threads} threads",
@@ -772,6 +993,11 @@ def main() -> int:
                     help="serve: tiles go to the oldest job (default: alternately the oldest and the least remaining work)")
     ap.add_argument("--serve-cpu-jobs", type=int, default=12, help="serve: served jobs checked against the oracle")
     ap.add_argument("--jobs", type=int, default=256, help="mixed: jobs drawn from the seeded generator")
+    ap.add_argument("--short-jobs", type=int, default=2000,
+                    help="mixed: also compare batched and plain on this many jobs with 2^24-wide windows (0: skip)")
+    ap.add_argument("--serve-block-jobs", type=int, default=600,
+                    help="mixed: jobs of the bounded serving stream (config 5 distribution; 0: skip)")
+    ap.add_argument("--serve-load", type=float, default=0.9, help="mixed: offered load of the serving stream")
     ap.add_argument("--tile-bits", type=int, default=0, help="mixed: log2 of the tile length (0 = engine default)")
     ap.add_argument("--batch-streams", type=int, default=0, help="mixed: streams of the batched form (0 = engine default)")
     ap.add_argument("--window-bits", type=int, default=30, help="mixed: log2 of each job's prime window")

@@ -53,3 +53,26 @@ def test_stride_forms_by_prime_size():
     assert bench.window_ops(t, "tribonacci", 1 << 40, 1 << 41) == 4.875
     assert bench.window_ops(t, "tribonacci", 1 << 45, 1 << 46) == 5.5
     assert bench.window_ops(t, "fibonacci", 1 << 40, 1 << 41, 5000) == t["loop"]["fibonacci"]["fp64_theta"]
+
+
+def test_contract_ops_follow_the_fixed_cost_table():
+    # W64 above 2^31: doubling 6, fixed-n 7, Fibonacci/Lucas 7, Tribonacci 13,
+    # Smarandache 40 (32 for power-of-2 bases); W32 below
+    big, small = 1 << 40, 1 << 30
+    assert [bench.contract_ops(s, 3, big) for s in (0, 1, 2)] == [6.0, 6.0, 6.0]
+    assert [bench.contract_ops(s, 3, small) for s in (0, 1, 2)] == [4.0, 4.0, 4.0]
+    assert bench.contract_ops(6, 100, big) == 7.0 and bench.contract_ops(7, 100, big) == 7.0
+    assert bench.contract_ops(4, 0, big) == 7.0 and bench.contract_ops(5, 0, big) == 7.0
+    assert bench.contract_ops(8, 0, big) == 13.0 and bench.contract_ops(8, 0, small) == 7.0
+    assert bench.contract_ops(3, 10, big) == 40.0 and bench.contract_ops(3, 16, big) == 32.0
+    assert bench.contract_ops(3, 2, big) == 32.0 and bench.contract_ops(3, 7, big) == 40.0
+
+
+def test_cpu_share_respects_affinity_and_omp(monkeypatch):
+    n_aff = len(os.sched_getaffinity(0))
+    monkeypatch.setenv("OMP_NUM_THREADS", "1")
+    n, src = bench.cpu_share()
+    assert n == 1 and "OMP_NUM_THREADS 1" in src and f"host logical CPUs {os.cpu_count()}" in src
+    monkeypatch.delenv("OMP_NUM_THREADS")
+    n, _ = bench.cpu_share()
+    assert 1 <= n <= n_aff

@@ -212,6 +212,31 @@ def test_cfg3_subwindow(engine):
     assert g == o.trial_factor(o.PROTH, 3, 1, 20000, f0, f1)
 
 
+def test_cfg3_proth_full_index_range(engine):
+    # config 3b's whole index range, Proth k=3 n in [1, 10^5), on [2^40, 2^40 + 2^18)
+    f0, f1 = 1 << 40, (1 << 40) + (1 << 18)
+    print("[cfg3] proth,3 n in [1, 100000)", flush=True)
+    g, _, _ = gpu_lists(engine, o.PROTH, 3, 1, 100000, f0, f1, mx=64)
+    assert g == o.trial_factor(o.PROTH, 3, 1, 100000, f0, f1, 64, threads=16)
+
+
+def test_cfg3_batched_beside_plain(engine):
+    # config 3's two jobs in the batched form (several tiles each) and in the
+    # plain form: both equal the oracle
+    specs = [(o.SMARANDACHE, 16, 1, 2000, 1 << 40, (1 << 40) + (1 << 22)),
+             (o.PROTH, 3, 1, 100000, 1 << 40, (1 << 40) + (1 << 18))]
+    mx = 64
+    jobs = [nat.CtdJob(s, mx, p, a, b, f0, f1) for s, p, a, b, f0, f1 in specs]
+    print("[cfg3] batched sm16 + proth3", flush=True)
+    res, st = engine.run_batch(jobs, tile_length=1 << 20, streams=2)
+    assert st.tiles == 4 + 1
+    for (s, p, a, b, f0, f1), (c, f) in zip(specs, res):
+        r = o.trial_factor(s, p, a, b, f0, f1, mx, threads=16)
+        assert ctd.FactoringResults.from_buffer(a, c, f, mx).factors == r, (s, p)
+        g, _, _ = gpu_lists(engine, s, p, a, b, f0, f1, mx=mx)
+        assert g == r, (s, p)
+
+
 def test_n0_zero_quirks(engine):
     # Fibonacci / Lucas starting at n0 = 0 (fibonacci.hpp:68-71): every prime
     # divides every index, so the counts overflow max_factors identically
