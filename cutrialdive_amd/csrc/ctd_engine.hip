@@ -183,6 +183,8 @@ struct ctd_hip_engine {
     uint64_t segoff_cap = 0;
     unsigned long long *d_scalars = nullptr; // [0] total, [1] max prime
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev_all0 = nullptr, ev_all1 = nullptr; // around a whole plain run
+    void *scratch = nullptr; // this device's allocation-free workspace (device address)
     ctd_run_stats stats{};
     // batched form
     std::vector<BatchSlot> slots;
@@ -243,84 +245,144 @@ constexpr uint64_t CHUNK_MAX = 1ull << 30;
 // length is rounded to whole segments and clamped to this.
 constexpr uint64_t BLOCK_SEGS_MAX = 1ull << 20;
 
-// Sieves [start, end) into an ordered list (u32 or u64) with the current
-// sieving numbers sp/mu.  Returns the count; allocates *out32 or *out64.
-uint64_t sieve_ordered(ctd_hip_engine *E, const uint32_t *sp, const uint64_t *mu, uint32_t nsp, uint32_t nsmall,
-                       uint32_t nwheel, uint64_t start, uint64_t end, uint32_t **out32, uint64_t **out64)
+// The sieving numbers of one sieve pass (ascending odd numbers; nwheel of
+// them <= WHEEL_MAX come from the wheel patterns, nsmall < SMALL_LIMIT).
+struct SieveBase {
+    const uint32_t *sp = nullptr;
+    const uint64_t *mu = nullptr;
+    uint32_t nsp = 0, nsmall = 0, nwheel = 0;
+};
+
+SieveArgs sieve_args(const SieveBase &B, uint64_t lo0, uint64_t start, uint64_t end, uint32_t mode)
 {
-    if (start >= end) return 0;
-    const uint64_t lo0 = start & ~1ull;
-    const uint64_t nseg = div_up(end - lo0, SEG_SPAN);
-    ensure_buf(E->d_segcnt, E->segcnt_cap, nseg);
-    ensure_buf(E->d_segoff, E->segoff_cap, nseg);
     SieveArgs A{};
     A.lo0 = lo0;
     A.start = start;
     A.end = end;
-    A.sp = sp;
-    A.sp_mu = mu;
-    A.nsp = nsp;
-    A.nsmall = nsmall;
-    A.nwheel = nwheel;
-    A.mode = SIEVE_COUNT;
-    A.seg_counts = E->d_segcnt;
-    A.seg_off = E->d_segoff;
-    hipLaunchKernelGGL(sieve_kernel, dim3((uint32_t)nseg), dim3(SIEVE_THREADS), 0, E->stream, A);
+    A.sp = B.sp;
+    A.sp_mu = B.mu;
+    A.nsp = B.nsp;
+    A.nsmall = B.nsmall;
+    A.nwheel = B.nwheel;
+    A.mode = mode;
+    return A;
+}
+
+// Count pass of an ordered sieve of [start, end): the per-segment prime
+// counts go to the host (h_cnt), their exclusive prefix sums to h_off and to
+// segoff on the device.  segcnt/segoff/h_cnt/h_off hold nseg entries, nseg =
+// the segments of [start & ~1, end).  Returns the total.
+uint64_t sieve_count_pass(hipStream_t st, const SieveBase &B, uint64_t start, uint64_t end, uint32_t *segcnt,
+                          uint64_t *segoff, uint32_t *h_cnt, uint64_t *h_off, uint64_t nseg)
+{
+    SieveArgs A = sieve_args(B, start & ~1ull, start, end, SIEVE_COUNT);
+    A.seg_counts = segcnt;
+    hipLaunchKernelGGL(sieve_kernel, dim3((uint32_t)nseg), dim3(SIEVE_THREADS), 0, st, A);
     HIPCHK(hipGetLastError());
-    std::vector<uint32_t> cnt(nseg);
-    HIPCHK(hipMemcpyAsync(cnt.data(), E->d_segcnt, nseg * sizeof(uint32_t), hipMemcpyDeviceToHost, E->stream));
-    HIPCHK(hipStreamSynchronize(E->stream));
-    std::vector<uint64_t> off(nseg);
+    HIPCHK(hipMemcpyAsync(h_cnt, segcnt, nseg * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
     uint64_t total = 0;
     for (uint64_t i = 0; i < nseg; ++i) {
-        off[i] = total;
-        total += cnt[i];
+        h_off[i] = total;
+        total += h_cnt[i];
     }
-    HIPCHK(hipMemcpyAsync(E->d_segoff, off.data(), nseg * sizeof(uint64_t), hipMemcpyHostToDevice, E->stream));
-    if (out32) {
-        HIPCHK(hipMalloc((void **)out32, std::max<uint64_t>(total, 1) * sizeof(uint32_t)));
-        A.out32 = *out32;
-        A.mode = SIEVE_ORDERED32;
-    } else {
-        HIPCHK(hipMalloc((void **)out64, std::max<uint64_t>(total, 1) * sizeof(uint64_t)));
-        A.out64 = *out64;
-        A.mode = SIEVE_ORDERED64;
-        A.cap = total;
-    }
-    hipLaunchKernelGGL(sieve_kernel, dim3((uint32_t)nseg), dim3(SIEVE_THREADS), 0, E->stream, A);
+    HIPCHK(hipMemcpyAsync(segoff, h_off, nseg * sizeof(uint64_t), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st)); // h_off may be reused once this returns
+    return total;
+}
+
+// Ordered pass after sieve_count_pass: the primes of [start, end) in
+// ascending order into out32 or out64 (the count pass's total entries).
+void sieve_order_pass(hipStream_t st, const SieveBase &B, uint64_t start, uint64_t end, uint32_t *segcnt,
+                      const uint64_t *segoff, uint64_t nseg, uint32_t *out32, uint64_t *out64, uint64_t total)
+{
+    SieveArgs A = sieve_args(B, start & ~1ull, start, end, out32 ? SIEVE_ORDERED32 : SIEVE_ORDERED64);
+    A.seg_counts = segcnt;
+    A.seg_off = segoff;
+    A.out32 = out32;
+    A.out64 = out64;
+    A.cap = total;
+    hipLaunchKernelGGL(sieve_kernel, dim3((uint32_t)nseg), dim3(SIEVE_THREADS), 0, st, A);
     HIPCHK(hipGetLastError());
+}
+
+uint64_t sieve_segments(uint64_t start, uint64_t end) { return start < end ? div_up(end - (start & ~1ull), SEG_SPAN) : 0; }
+
+// Sieves [start, end) into an ordered list (u32 or u64) allocated here.
+uint64_t sieve_ordered(ctd_hip_engine *E, const SieveBase &B, uint64_t start, uint64_t end, uint32_t **out32,
+                       uint64_t **out64)
+{
+    if (start >= end) return 0;
+    const uint64_t nseg = sieve_segments(start, end);
+    ensure_buf(E->d_segcnt, E->segcnt_cap, nseg);
+    ensure_buf(E->d_segoff, E->segoff_cap, nseg);
+    std::vector<uint32_t> cnt(nseg);
+    std::vector<uint64_t> off(nseg);
+    const uint64_t total = sieve_count_pass(E->stream, B, start, end, E->d_segcnt, E->d_segoff, cnt.data(),
+                                            off.data(), nseg);
+    if (out32) HIPCHK(hipMalloc((void **)out32, std::max<uint64_t>(total, 1) * sizeof(uint32_t)));
+    else HIPCHK(hipMalloc((void **)out64, std::max<uint64_t>(total, 1) * sizeof(uint64_t)));
+    sieve_order_pass(E->stream, B, start, end, E->d_segcnt, E->d_segoff, nseg, out32 ? *out32 : nullptr,
+                     out32 ? nullptr : *out64, total);
     HIPCHK(hipStreamSynchronize(E->stream));
     return total;
 }
 
-// Entries of an ascending sieving list below `limit` (at most 4096 looked at).
-uint32_t count_below(ctd_hip_engine *E, const uint32_t *d_list, uint32_t n, uint32_t limit)
+// The small and wheel counts of an ascending sieving list of n numbers;
+// h holds the first min(n, 4096) of them on the host.
+void count_small(const uint32_t *h, uint32_t n, uint32_t &nsmall, uint32_t &nwheel)
 {
-    uint32_t k = std::min<uint32_t>(n, 4096);
-    std::vector<uint32_t> h(k);
-    if (k) HIPCHK(hipMemcpy(h.data(), d_list, k * sizeof(uint32_t), hipMemcpyDeviceToHost));
-    uint32_t s = 0;
-    while (s < k && h[s] < limit) ++s;
-    return s;
-}
-
-// The small and wheel counts of a sieving list, checked against the sieve's
-// LDS phase table.
-void count_small(ctd_hip_engine *E, const uint32_t *d_list, uint32_t n, uint32_t &nsmall, uint32_t &nwheel)
-{
-    nsmall = count_below(E, d_list, n, SMALL_LIMIT);
-    nwheel = count_below(E, d_list, n, WHEEL_MAX + 1);
+    const uint32_t k = std::min<uint32_t>(n, 4096);
+    nsmall = nwheel = 0;
+    while (nsmall < k && h[nsmall] < SMALL_LIMIT) ++nsmall;
+    while (nwheel < k && h[nwheel] <= WHEEL_MAX) ++nwheel;
     if (nsmall - nwheel > SMALL_MAX_COUNT) throw HipError{hipErrorInvalidValue, "sieving list: too many small numbers"};
 }
 
-void make_mu(ctd_hip_engine *E, const uint32_t *sp, uint64_t *mu, uint32_t n)
+void list_counts(hipStream_t st, const uint32_t *d_list, uint32_t n, uint32_t *h4096, SieveBase &B)
+{
+    const uint32_t k = std::min<uint32_t>(n, 4096);
+    if (k) HIPCHK(hipMemcpyAsync(h4096, d_list, k * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    B.sp = d_list;
+    B.nsp = n;
+    count_small(h4096, n, B.nsmall, B.nwheel);
+}
+
+void make_mu(hipStream_t st, const uint32_t *sp, uint64_t *mu, uint32_t n)
 {
     if (!n) return;
-    hipLaunchKernelGGL(sieve_mu_kernel, dim3(div_up(n, 256)), dim3(256), 0, E->stream, sp, mu, n);
+    hipLaunchKernelGGL(sieve_mu_kernel, dim3(div_up(n, 256)), dim3(256), 0, st, sp, mu, n);
     HIPCHK(hipGetLastError());
 }
 
-// Odd primes <= lim (lim < 2^32) as the sieving base.
+// Odd primes below 65536 sieve every window up to 2^32: they are the
+// second level of every sieving base.
+constexpr uint32_t L0_COUNT = 127;   // odd numbers 3..255 (level 0)
+constexpr uint32_t L1_CAP = 6542;    // odd primes below 65536: 6541
+constexpr uint64_t L1_END = 65537;
+
+// Levels 0 and 1 of a sieving base into l0/m0 [L0_COUNT] and l1/m1 [L1_CAP]:
+// the odd numbers 3..255 sieve [3, 65537) into the odd primes below 65536.
+// segcnt/segoff/h_cnt/h_off need one entry.
+SieveBase build_level1(hipStream_t st, uint32_t *l0, uint64_t *m0, uint32_t *l1, uint64_t *m1, uint32_t *segcnt,
+                       uint64_t *segoff, uint32_t *h_cnt, uint64_t *h_off, uint32_t *h4096)
+{
+    hipLaunchKernelGGL(odd_numbers_kernel, dim3(1), dim3(128), 0, st, l0, L0_COUNT);
+    make_mu(st, l0, m0, L0_COUNT);
+    SieveBase B0{l0, m0, L0_COUNT, L0_COUNT, (WHEEL_MAX - 1) / 2};
+    const uint64_t nseg = sieve_segments(3, L1_END);
+    const uint64_t n1 = sieve_count_pass(st, B0, 3, L1_END, segcnt, segoff, h_cnt, h_off, nseg);
+    if (n1 > L1_CAP) throw HipError{hipErrorInvalidValue, "sieving base: level-1 list overflow"};
+    sieve_order_pass(st, B0, 3, L1_END, segcnt, segoff, nseg, l1, nullptr, n1);
+    make_mu(st, l1, m1, (uint32_t)n1);
+    SieveBase B1;
+    B1.mu = m1;
+    list_counts(st, l1, (uint32_t)n1, h4096, B1);
+    return B1;
+}
+
+// Odd primes <= lim (lim < 2^32) as the engine's sieving base.
 void build_base(ctd_hip_engine *E, uint64_t lim)
 {
     if (lim < 65536) lim = 65536;
@@ -329,40 +391,41 @@ void build_base(ctd_hip_engine *E, uint64_t lim)
     // little each time, and sieving numbers beyond sqrt(end) cost a segment
     // nothing (the per-lane loop stops at the first one past it)
     if (E->d_sp) lim = std::min<uint64_t>(std::max<uint64_t>(lim, 2 * E->base_lim), 0xffffffffull);
-    // level 0: odd numbers 3..255 sieve [0, 65537)
-    uint32_t *l0 = nullptr;
-    uint64_t *m0 = nullptr;
-    const uint32_t n0 = 127;
-    HIPCHK(hipMalloc((void **)&l0, n0 * sizeof(uint32_t)));
-    HIPCHK(hipMalloc((void **)&m0, n0 * sizeof(uint64_t)));
-    hipLaunchKernelGGL(odd_numbers_kernel, dim3(1), dim3(128), 0, E->stream, l0, n0);
-    make_mu(E, l0, m0, n0);
-    uint32_t *l1 = nullptr;
-    uint32_t n1 = (uint32_t)sieve_ordered(E, l0, m0, n0, n0, (WHEEL_MAX - 1) / 2, 3, 65537, &l1, nullptr);
+    uint32_t *l0 = nullptr, *l1 = nullptr, *l2 = nullptr;
+    uint64_t *m0 = nullptr, *m1 = nullptr, *m2 = nullptr;
+    HIPCHK(hipMalloc((void **)&l0, L0_COUNT * sizeof(uint32_t)));
+    HIPCHK(hipMalloc((void **)&m0, L0_COUNT * sizeof(uint64_t)));
+    HIPCHK(hipMalloc((void **)&l1, L1_CAP * sizeof(uint32_t)));
+    HIPCHK(hipMalloc((void **)&m1, L1_CAP * sizeof(uint64_t)));
+    ensure_buf(E->d_segcnt, E->segcnt_cap, 1);
+    ensure_buf(E->d_segoff, E->segoff_cap, 1);
+    std::vector<uint32_t> h(4096);
+    uint32_t hc;
+    uint64_t ho;
+    const SieveBase B1 = build_level1(E->stream, l0, m0, l1, m1, E->d_segcnt, E->d_segoff, &hc, &ho, h.data());
+    // level 2: primes below 65536 sieve [3, lim + 1)
+    const uint64_t n2 = sieve_ordered(E, B1, 3, lim + 1, &l2, nullptr);
+    HIPCHK(hipMalloc((void **)&m2, std::max<uint64_t>(n2, 1) * sizeof(uint64_t)));
+    make_mu(E->stream, l2, m2, (uint32_t)n2);
+    HIPCHK(hipStreamSynchronize(E->stream));
     HIPCHK(hipFree(l0));
     HIPCHK(hipFree(m0));
-    uint64_t *m1 = nullptr;
-    HIPCHK(hipMalloc((void **)&m1, n1 * sizeof(uint64_t)));
-    make_mu(E, l1, m1, n1);
-    uint32_t s1, w1;
-    count_small(E, l1, n1, s1, w1);
-    // level 1: primes <= 65536 sieve [3, lim + 1)
-    uint32_t *l2 = nullptr;
-    uint32_t n2 = (uint32_t)sieve_ordered(E, l1, m1, n1, s1, w1, 3, lim + 1, &l2, nullptr);
     HIPCHK(hipFree(l1));
     HIPCHK(hipFree(m1));
-    uint64_t *m2 = nullptr;
-    HIPCHK(hipMalloc((void **)&m2, std::max<uint32_t>(n2, 1) * sizeof(uint64_t)));
-    make_mu(E, l2, m2, n2);
-    HIPCHK(hipStreamSynchronize(E->stream));
     if (E->d_sp) HIPCHK(hipFree(E->d_sp));
     if (E->d_mu) HIPCHK(hipFree(E->d_mu));
     E->d_sp = l2;
     E->d_mu = m2;
-    E->nsp = n2;
-    count_small(E, l2, n2, E->nsmall, E->nwheel);
+    SieveBase B;
+    B.mu = m2;
+    list_counts(E->stream, l2, (uint32_t)n2, h.data(), B);
+    E->nsp = B.nsp;
+    E->nsmall = B.nsmall;
+    E->nwheel = B.nwheel;
     E->base_lim = lim;
 }
+
+SieveBase engine_base(const ctd_hip_engine *E) { return SieveBase{E->d_sp, E->d_mu, E->nsp, E->nsmall, E->nwheel}; }
 
 unsigned bitlen64(uint64_t x) { return x ? 64u - (unsigned)__builtin_clzll(x) : 0u; }
 
@@ -438,9 +501,27 @@ struct HostSync {
     uint64_t *h_factors;
 };
 
+// Device buffers of the segment loop: the sieving base, one batch's prime
+// stream, its per-segment counts and the batch's two scalars (count, largest
+// prime).  The engine's own buffers grow on demand; a fixed workspace (the
+// allocation-free entry point's) is used as it is, with batches of at most
+// batch_max integers.
+struct Workspace {
+    SieveBase base;
+    uint64_t *primes = nullptr;
+    uint64_t primes_cap = 0;
+    uint32_t *segcnt = nullptr;
+    uint64_t segcnt_cap = 0;
+    unsigned long long *scalars = nullptr;
+    uint64_t batch_max = ~0ull;
+};
+
 // The segment loop.  d_counts/d_factors hold the job's factors buffer.
+// fixed: the workspace to use (its base covers sqrt(f1)), or null for the
+// engine's own buffers.
 int run_job(ctd_hip_engine *E, const ctd_job *job, const ctd_run_opts *opts, const ctd_callbacks *cb,
-            uint32_t *d_counts, uint64_t *d_factors, uint64_t *last_prime, hipStream_t st, const HostSync *hs)
+            uint32_t *d_counts, uint64_t *d_factors, uint64_t *last_prime, hipStream_t st, const HostSync *hs,
+            const Workspace *fixed = nullptr)
 {
     int code;
     if (!valid_job(E, job, &code)) return code;
@@ -465,10 +546,7 @@ int run_job(ctd_hip_engine *E, const ctd_job *job, const ctd_run_opts *opts, con
     J.counts = d_counts;
     J.factors = d_factors;
 
-    hipEvent_t t_all0, t_all1;
-    HIPCHK(hipEventCreate(&t_all0));
-    HIPCHK(hipEventCreate(&t_all1));
-    HIPCHK(hipEventRecord(t_all0, st));
+    HIPCHK(hipEventRecord(E->ev_all0, st));
 
     uint64_t f0 = job->f0;
     const uint64_t f1 = job->f1;
@@ -483,11 +561,18 @@ int run_job(ctd_hip_engine *E, const ctd_job *job, const ctd_run_opts *opts, con
     }
     uint64_t lastp = (job->f0 <= 2 && f1 > 2) ? 2 : 0;
     if (f0 < f1) {
-        build_base(E, isqrt_u64(f1 - 1));
+        Workspace ws;
+        if (fixed) {
+            ws = *fixed;
+        } else {
+            build_base(E, isqrt_u64(f1 - 1));
+            ws.base = engine_base(E);
+        }
         const bool generic_only = opts && (opts->flags & CTD_RUN_GENERIC_ONLY);
         const bool fast = !generic_only && Fast<0>::fits(f1 - 1);
-        const uint64_t blen = whole_spans((opts && opts->batch_length) ? opts->batch_length : (1ull << 31), SEG_SPAN,
-                                          BLOCK_SEGS_MAX);
+        const uint64_t blen = std::min(whole_spans((opts && opts->batch_length) ? opts->batch_length : (1ull << 31),
+                                                   SEG_SPAN, BLOCK_SEGS_MAX),
+                                       ws.batch_max);
         const uint64_t lo0 = f0 & ~1ull;
         for (uint64_t blo = lo0; blo < f1;) {
             const uint64_t bhi = block_end(blo, blen, f1);
@@ -497,38 +582,38 @@ int run_job(ctd_hip_engine *E, const ctd_job *job, const ctd_run_opts *opts, con
             // Montgomery-Vaughan: pi(x+y) - pi(x) <= 2y / log y
             double ly = std::log((double)std::max<uint64_t>(len, 16));
             uint64_t cap = (uint64_t)(2.0 * (double)len / ly) + 4096;
-            ensure_buf(E->d_primes, E->primes_cap, cap);
-            ensure_buf(E->d_segcnt, E->segcnt_cap, nseg);
-            if (!E->d_scalars) HIPCHK(hipMalloc((void **)&E->d_scalars, 2 * sizeof(unsigned long long)));
-            hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, st, E->d_scalars, 2u);
-            SieveArgs A{};
-            A.lo0 = blo;
-            A.start = bstart;
-            A.end = bhi;
-            A.sp = E->d_sp;
-            A.sp_mu = E->d_mu;
-            A.nsp = E->nsp;
-            A.nsmall = E->nsmall;
-            A.nwheel = E->nwheel;
-            A.mode = SIEVE_DENSE;
-            A.seg_counts = E->d_segcnt;
-            A.out64 = E->d_primes;
-            A.total = E->d_scalars;
-            A.maxp = E->d_scalars + 1;
-            A.cap = E->primes_cap;
+            if (!fixed) {
+                ensure_buf(E->d_primes, E->primes_cap, cap);
+                ensure_buf(E->d_segcnt, E->segcnt_cap, nseg);
+                if (!E->d_scalars) HIPCHK(hipMalloc((void **)&E->d_scalars, 2 * sizeof(unsigned long long)));
+                ws.primes = E->d_primes;
+                ws.primes_cap = E->primes_cap;
+                ws.segcnt = E->d_segcnt;
+                ws.segcnt_cap = E->segcnt_cap;
+                ws.scalars = E->d_scalars;
+            } else if (nseg > ws.segcnt_cap) {
+                return fail(E, CTD_E_INVALID, "batch of " + std::to_string(len) + " integers exceeds the workspace");
+            }
+            hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, st, ws.scalars, 2u);
+            SieveArgs A = sieve_args(ws.base, blo, bstart, bhi, SIEVE_DENSE);
+            A.seg_counts = ws.segcnt;
+            A.out64 = ws.primes;
+            A.total = ws.scalars;
+            A.maxp = ws.scalars + 1;
+            A.cap = ws.primes_cap;
             HIPCHK(hipEventRecord(E->ev0, st));
             hipLaunchKernelGGL(sieve_kernel, dim3((uint32_t)nseg), dim3(SIEVE_THREADS), 0, st, A);
             HIPCHK(hipGetLastError());
             HIPCHK(hipEventRecord(E->ev1, st));
             unsigned long long sc[2];
-            HIPCHK(hipMemcpyAsync(sc, E->d_scalars, sizeof sc, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipMemcpyAsync(sc, ws.scalars, sizeof sc, hipMemcpyDeviceToHost, st));
             HIPCHK(hipStreamSynchronize(st));
             float ms = 0;
             HIPCHK(hipEventElapsedTime(&ms, E->ev0, E->ev1));
             E->stats.sieve_ms += ms;
             E->stats.sieve_launches++;
             uint64_t np = sc[0];
-            if (np > E->primes_cap)
+            if (np > ws.primes_cap)
                 return fail(E, CTD_E_DEVICE, "prime buffer overflow in batch starting at " + std::to_string(blo));
             if (np) {
                 // index chunking so that small prime batches still fill the device
@@ -559,7 +644,7 @@ int run_job(ctd_hip_engine *E, const ctd_job *job, const ctd_run_opts *opts, con
                 uint64_t grid = (opts && opts->grid_size > 0) ? (uint64_t)opts->grid_size : div_up(items, block);
                 grid = std::min<uint64_t>(grid, 1ull << 30);
                 HIPCHK(hipEventRecord(E->ev0, st));
-                launch_tf_any(J, E->d_primes, np, nullptr, fast, dim3((uint32_t)grid), dim3(block), st);
+                launch_tf_any(J, ws.primes, np, nullptr, fast, dim3((uint32_t)grid), dim3(block), st);
                 HIPCHK(hipGetLastError());
                 HIPCHK(hipEventRecord(E->ev1, st));
                 HIPCHK(hipStreamSynchronize(st));
@@ -582,13 +667,11 @@ int run_job(ctd_hip_engine *E, const ctd_job *job, const ctd_run_opts *opts, con
             blo = bhi;
         }
     }
-    HIPCHK(hipEventRecord(t_all1, st));
+    HIPCHK(hipEventRecord(E->ev_all1, st));
     HIPCHK(hipStreamSynchronize(st));
     float tot = 0;
-    HIPCHK(hipEventElapsedTime(&tot, t_all0, t_all1));
+    HIPCHK(hipEventElapsedTime(&tot, E->ev_all0, E->ev_all1));
     E->stats.total_ms = tot;
-    HIPCHK(hipEventDestroy(t_all0));
-    HIPCHK(hipEventDestroy(t_all1));
     if (last_prime) *last_prime = lastp;
     return CTD_OK;
 }
@@ -628,6 +711,8 @@ int ctd_hip_open(int device, ctd_hip_engine **out)
         HIPCHK(hipStreamCreateWithFlags(&E->stream, hipStreamNonBlocking));
         HIPCHK(hipEventCreate(&E->ev0));
         HIPCHK(hipEventCreate(&E->ev1));
+        HIPCHK(hipEventCreate(&E->ev_all0));
+        HIPCHK(hipEventCreate(&E->ev_all1));
     } catch (const HipError &h) {
         g_open_error = std::string(h.what) + ": " + hipGetErrorString(h.code);
         delete E;
@@ -651,6 +736,8 @@ void ctd_hip_close(ctd_hip_engine *E)
     release_batch(E);
     if (E->ev0) (void)hipEventDestroy(E->ev0);
     if (E->ev1) (void)hipEventDestroy(E->ev1);
+    if (E->ev_all0) (void)hipEventDestroy(E->ev_all0);
+    if (E->ev_all1) (void)hipEventDestroy(E->ev_all1);
     if (E->stream) (void)hipStreamDestroy(E->stream);
     delete E;
 }
@@ -688,7 +775,7 @@ int ctd_hip_sieve(ctd_hip_engine *E, uint64_t start, uint64_t end, uint64_t *out
         uint64_t *d = nullptr;
         if (s < end) {
             build_base(E, isqrt_u64(end - 1));
-            n = sieve_ordered(E, E->d_sp, E->d_mu, E->nsp, E->nsmall, E->nwheel, s, end, nullptr, &d);
+            n = sieve_ordered(E, engine_base(E), s, end, nullptr, &d);
         }
         *count = n + n2;
         if (out && cap) {
