@@ -163,6 +163,18 @@ SIGNATURES = {
             ctypes.c_void_p,
         ],
     ),
+    "ctd_hip_trial_factor_device_noalloc": (
+        ctypes.c_int,
+        [
+            ctypes.c_void_p,
+            ctypes.POINTER(CtdJob),
+            ctypes.POINTER(CtdRunOpts),
+            ctypes.c_void_p,
+            ctypes.c_void_p,
+            ctypes.POINTER(u64),
+            ctypes.c_void_p,
+        ],
+    ),
     "ctd_hip_last_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(CtdRunStats)]),
     "ctd_hip_residues": (
         ctypes.c_int,

@@ -679,6 +679,85 @@ int run_job(ctd_hip_engine *E, const ctd_job *job, const ctd_run_opts *opts, con
 #include "ctd_batch.hpp"
 #include "ctd_serve.hpp"
 
+// ---- the allocation-free workspace -----------------------------------------
+// One fixed device array per GPU holds everything the plain form allocates
+// for itself: the sieving base (odd primes below 2^24, enough for prime
+// windows below 2^48), one batch's prime stream (batches of 2^25 integers:
+// at most 2 * 2^25 / ln 2^25 + 4096 < 2^22 primes, Montgomery-Vaughan), the
+// per-segment counts and the batch scalars.
+constexpr uint64_t NA_F1_MAX = 1ull << 48;
+constexpr uint32_t NA_BASE_CAP = (1u << 20) + (1u << 15); // >= 1,077,870 odd primes below 2^24
+constexpr uint64_t NA_BATCH = 1ull << 25;
+constexpr uint64_t NA_PRIME_CAP = 1ull << 22;
+constexpr uint64_t NA_SEGS = 64; // segments of one pass at most: 32 per batch, 16 for the base
+constexpr uint64_t na_align(uint64_t x) { return (x + 255) & ~255ull; }
+constexpr uint64_t NA_L0 = 0;
+constexpr uint64_t NA_M0 = na_align(NA_L0 + L0_COUNT * 4);
+constexpr uint64_t NA_L1 = na_align(NA_M0 + L0_COUNT * 8);
+constexpr uint64_t NA_M1 = na_align(NA_L1 + L1_CAP * 4);
+constexpr uint64_t NA_SP = na_align(NA_M1 + L1_CAP * 8);
+constexpr uint64_t NA_MU = na_align(NA_SP + NA_BASE_CAP * 4ull);
+constexpr uint64_t NA_SEGCNT = na_align(NA_MU + NA_BASE_CAP * 8ull);
+constexpr uint64_t NA_SEGOFF = na_align(NA_SEGCNT + NA_SEGS * 4);
+constexpr uint64_t NA_SCALARS = na_align(NA_SEGOFF + NA_SEGS * 8);
+constexpr uint64_t NA_PRIMES = na_align(NA_SCALARS + 2 * 8);
+constexpr uint64_t NA_BYTES = na_align(NA_PRIMES + NA_PRIME_CAP * 8);
+static_assert(NA_BYTES == CTD_NOALLOC_WORKSPACE_BYTES, "workspace size differs from the header's");
+static_assert(NA_BATCH / SEG_SPAN + 1 <= NA_SEGS && (1ull << 24) / SEG_SPAN + 1 <= NA_SEGS, "segment counts");
+
+// The array itself.  In host code its name is only a stand-in for the
+// symbol: kernels get the device address hipGetSymbolAddress returns for it
+// (g_scratch, per device), never the name.
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+std::mutex g_scratch_mu;
+void *g_scratch[CTD_MAX_DEVICES] = {};
+
+// This device's workspace address (the device is current), looked up once.
+void *scratch_address(int device)
+{
+    if (device < 0 || device >= CTD_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[device]) {
+        void *p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch[device] = p;
+    }
+    return g_scratch[device];
+}
+
+// The workspace of the allocation-free form at device address W, with the
+// odd primes <= max(lim, 65536) (lim < 2^24) built into it.  Allocates
+// nothing: the host side uses arrays on the stack.
+Workspace scratch_workspace(hipStream_t st, unsigned char *W, uint64_t lim)
+{
+    if (lim < 65536) lim = 65536;
+    uint32_t h_cnt[NA_SEGS];
+    uint64_t h_off[NA_SEGS];
+    uint32_t h4096[4096];
+    Workspace ws;
+    ws.segcnt = (uint32_t *)(W + NA_SEGCNT);
+    ws.segcnt_cap = NA_SEGS;
+    uint64_t *segoff = (uint64_t *)(W + NA_SEGOFF);
+    const SieveBase B1 = build_level1(st, (uint32_t *)(W + NA_L0), (uint64_t *)(W + NA_M0), (uint32_t *)(W + NA_L1),
+                                      (uint64_t *)(W + NA_M1), ws.segcnt, segoff, h_cnt, h_off, h4096);
+    const uint64_t nseg = sieve_segments(3, lim + 1);
+    if (nseg > NA_SEGS) throw HipError{hipErrorInvalidValue, "sieving base beyond the workspace"};
+    uint32_t *sp = (uint32_t *)(W + NA_SP);
+    uint64_t *mu = (uint64_t *)(W + NA_MU);
+    const uint64_t n = sieve_count_pass(st, B1, 3, lim + 1, ws.segcnt, segoff, h_cnt, h_off, nseg);
+    if (n > NA_BASE_CAP) throw HipError{hipErrorInvalidValue, "sieving base beyond the workspace"};
+    sieve_order_pass(st, B1, 3, lim + 1, ws.segcnt, segoff, nseg, sp, nullptr, n);
+    make_mu(st, sp, mu, (uint32_t)n);
+    ws.base.mu = mu;
+    list_counts(st, sp, (uint32_t)n, h4096, ws.base);
+    ws.primes = (uint64_t *)(W + NA_PRIMES);
+    ws.primes_cap = NA_PRIME_CAP;
+    ws.scalars = (unsigned long long *)(W + NA_SCALARS);
+    ws.batch_max = NA_BATCH;
+    return ws;
+}
+
 } // namespace
 
 // ---------------------------------------------------------------------------
@@ -713,6 +792,7 @@ int ctd_hip_open(int device, ctd_hip_engine **out)
         HIPCHK(hipEventCreate(&E->ev1));
         HIPCHK(hipEventCreate(&E->ev_all0));
         HIPCHK(hipEventCreate(&E->ev_all1));
+        E->scratch = scratch_address(device);
     } catch (const HipError &h) {
         g_open_error = std::string(h.what) + ": " + hipGetErrorString(h.code);
         delete E;
@@ -996,6 +1076,30 @@ int ctd_hip_trial_factor_device(ctd_hip_engine *E, const ctd_job *job, const ctd
         HIPCHK(hipSetDevice(E->device));
         hipStream_t st = stream ? (hipStream_t)stream : E->stream;
         return run_job(E, job, opts, nullptr, d_counts, d_factors, last_prime, st, nullptr);
+    } catch (const HipError &h) {
+        return fail_hip(E, h);
+    }
+}
+
+int ctd_hip_trial_factor_device_noalloc(ctd_hip_engine *E, const ctd_job *job, const ctd_run_opts *opts,
+                                        uint32_t *d_counts, uint64_t *d_factors, uint64_t *last_prime, void *stream)
+{
+    if (!E) return CTD_E_INVALID;
+    int code;
+    if (!valid_job(E, job, &code)) return code;
+    if (job->f1 > CTD_NOALLOC_F1_MAX)
+        return fail(E, CTD_E_UNSUPPORTED, "the allocation-free form takes prime windows up to 2^48 (its fixed "
+                                          "workspace holds the sieving primes below 2^24); f1 = " +
+                                              std::to_string(job->f1));
+    if (!E->scratch) return fail(E, CTD_E_DEVICE, "no allocation-free workspace on this device");
+    try {
+        HIPCHK(hipSetDevice(E->device));
+        hipStream_t st = stream ? (hipStream_t)stream : E->stream;
+        Workspace ws;
+        const uint64_t f0 = std::max<uint64_t>(job->f0, 3);
+        if (job->n1 > job->n0 && f0 < job->f1)
+            ws = scratch_workspace(st, (unsigned char *)E->scratch, isqrt_u64(job->f1 - 1));
+        return run_job(E, job, opts, nullptr, d_counts, d_factors, last_prime, st, nullptr, &ws);
     } catch (const HipError &h) {
         return fail_hip(E, h);
     }

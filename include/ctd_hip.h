@@ -127,6 +127,28 @@ int ctd_hip_trial_factor_device(ctd_hip_engine *eng, const ctd_job *job,
                                 uint64_t *d_factors, uint64_t *last_prime,
                                 void *stream);
 
+/* Allocation-free form of ctd_hip_trial_factor_device: the same arguments and
+ * results, and no device memory is allocated or freed during the call.  No
+ * counterpart in the reference; it is for callers that cannot allocate:
+ * code being captured into a HIP graph, tight serving loops, harnesses that
+ * forbid hipMalloc.  Its sieving base, prime stream, segment counts and
+ * scalars live in one fixed device array of CTD_NOALLOC_WORKSPACE_BYTES
+ * (about 46 MiB) per GPU, which each GPU the library runs on holds until the
+ * process ends; ctd_hip_open looks up its address.  Limits: f1 <=
+ * CTD_NOALLOC_F1_MAX = 2^48 (the array holds the sieving primes below 2^24;
+ * larger windows return CTD_E_UNSUPPORTED), and device batches of at most
+ * 2^25 integers (opts->batch_length above that is capped).  Every call on a
+ * GPU uses that GPU's one array and leaves its contents behind: calls that
+ * overlap on one GPU (from two threads, or on two streams) race and must not
+ * be made. */
+#define CTD_NOALLOC_F1_MAX ((uint64_t)1 << 48)
+#define CTD_NOALLOC_WORKSPACE_BYTES ((uint64_t)46611968)
+#define CTD_MAX_DEVICES 64
+int ctd_hip_trial_factor_device_noalloc(ctd_hip_engine *eng, const ctd_job *job,
+                                        const ctd_run_opts *opts, uint32_t *d_counts,
+                                        uint64_t *d_factors, uint64_t *last_prime,
+                                        void *stream);
+
 /* Statistics of the last run on this engine: primes tested (odd primes of the
  * window), residue updates (primes x indices), kernel milliseconds of the
  * trial-division kernels and of the sieve kernels (HIP events). */
