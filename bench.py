@@ -93,6 +93,8 @@ def loop_ops(t, fam: str, pmax: int, pmin: int, n_items: int = 0) -> float:
     (ctd_tf_fast.hpp use_fp, ctd_tf_fp.hpp); the Smarandache FP64 form also
     needs p > 8 n1, which every bench job meets."""
     e = t["loop"][fam]
+    if "closed_per_prime" in e and n_items:  # one solve per prime, whatever the index count
+        return float(e["closed_per_prime"]["fp64" if pmax < (1 << 48) else "int"]) / n_items
     if pmax < (1 << 48):
         if fam in ("fibonacci", "lucas") and n_items >= 1024:
             return float(e["fp64_theta"])

@@ -26,10 +26,12 @@ CTD_E_BUSY = -5
 CTD_RUN_GENERIC_ONLY = 1
 CTD_RUN_NO_FP64 = 2
 CTD_RUN_SM_FULL_STEP = 4
+CTD_RUN_STEPPED = 8
 CTD_BATCH_HOST_BUFFERS = 1
 CTD_BATCH_NO_GRAPH = 2
 CTD_BATCH_NO_AUTOTUNE = 4
 CTD_BATCH_NO_FP64 = 8
+CTD_BATCH_STEPPED = 16
 CTD_SERVE_FRESH_BUFFERS = 1
 CTD_SERVE_FIFO = 2
 

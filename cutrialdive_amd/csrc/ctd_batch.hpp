@@ -95,6 +95,7 @@ struct JobPlan {
     LaunchShape shape;
     double ops = 7.0;  // relative cost per update, for ordering tiles
     bool fp64 = true;  // the FP64 forms are allowed
+    uint32_t stepped = 0; // JOB_STEPPED when the fixed-n families step every index
 };
 
 struct Tile {
@@ -114,9 +115,11 @@ constexpr uint32_t SERVE_LANES_PER_CU = 4096;
 
 // VALU instructions per update of the family's loop (FP64 forms below 2^48,
 // integer forms above), for ordering tiles and balancing the streams.
-double family_cost(uint32_t fam, bool fast, bool fp64, uint64_t N)
+double family_cost(uint32_t fam, bool fast, bool fp64, uint64_t N, uint32_t stepped = 0)
 {
     if (!fast) return 50.0;
+    // one power of about log2(n) multiplies per prime, whatever the index count
+    if (closed_form(fam, stepped)) return (fp64 ? 300.0 : 2000.0) / (double)std::max<uint64_t>(N, 1);
     switch (fam) {
     case FAM_SMARANDACHE: return fp64 ? 7.0 : 12.25;
     case FAM_TRIBONACCI: return fp64 ? 5.5 : 11.25;
@@ -139,12 +142,14 @@ JobArgs tile_args(const ctd_hip_engine *E, const JobPlan &P, uint64_t est_np, ui
     if (est_np < target && P.N > 2048) nch = std::min<uint64_t>(div_up(target, std::max<uint64_t>(est_np, 1)),
                                                                  div_up(P.N, 1024));
     nch = std::min<uint64_t>(std::max<uint64_t>(nch * P.shape.split, div_up(P.N, CHUNK_MAX)), P.N);
+    if (closed_form(J.fam, P.stepped)) nch = 1; // one solve per prime covers every index
     const uint64_t chunk = div_up(P.N, nch);
     J.chunk = chunk;
     J.nchunks = (uint32_t)div_up(P.N, chunk);
     J.flags = (J.fam == FAM_SMARANDACHE && sm_narrow_ok(J.param, J.n1, pmax)) ? JOB_SM_NARROW : 0u;
     if (pmax < (1ull << 37)) J.flags |= JOB_P37;
     if (pmax < FP_PMAX && P.fp64) J.flags |= JOB_FP64;
+    J.flags |= P.stepped;
     return J;
 }
 
@@ -156,10 +161,11 @@ dim3 tile_grid(const ctd_hip_engine *E, const LaunchShape &sh, uint64_t est_np, 
     return dim3((uint32_t)std::max<uint64_t>(std::min<uint64_t>(g, 1ull << 30), 1));
 }
 
-uint64_t shape_key(uint32_t fam, bool fast, bool narrow, uint64_t N)
+uint64_t shape_key(uint32_t fam, bool fast, bool narrow, uint64_t N, uint32_t stepped = 0)
 {
     uint32_t ncls = N < 1024 ? 0 : N < 16384 ? 1 : 2;
-    return ((uint64_t)fam << 8) | ((uint64_t)fast << 4) | ((uint64_t)narrow << 3) | ncls;
+    return ((uint64_t)fam << 8) | ((uint64_t)(stepped != 0) << 5) | ((uint64_t)fast << 4) | ((uint64_t)narrow << 3) |
+           ncls;
 }
 
 void ensure_scratch(ctd_hip_engine *E, uint64_t n)
@@ -283,7 +289,8 @@ LaunchShape tune_shape(ctd_hip_engine *E, const JobPlan &P, bool fast)
 }
 
 int plan_jobs(ctd_hip_engine *E, const ctd_job *jobs, uint32_t njobs, uint32_t *const *counts,
-              uint64_t *const *factors, bool autotune, std::vector<JobPlan> &plans, uint32_t *tuned)
+              uint64_t *const *factors, bool autotune, std::vector<JobPlan> &plans, uint32_t *tuned,
+              uint32_t batch_flags = 0)
 {
     plans.resize(njobs);
     uint64_t fmax = 0;
@@ -307,16 +314,18 @@ int plan_jobs(ctd_hip_engine *E, const ctd_job *jobs, uint32_t njobs, uint32_t *
         P.has2 = P.N && q.f0 <= 2 && q.f1 > 2;
         P.f0 = std::max<uint64_t>(q.f0, 3);
         P.f1 = P.N ? q.f1 : 0;
+        P.fp64 = !(batch_flags & CTD_BATCH_NO_FP64);
+        P.stepped = (batch_flags & CTD_BATCH_STEPPED) ? JOB_STEPPED : 0u;
         if (P.f0 < P.f1) fmax = std::max(fmax, P.f1 - 1);
     }
     if (fmax) build_base(E, isqrt_u64(fmax));
     for (auto &P : plans) {
         if (P.f0 >= P.f1) continue;
         const bool fast = Fast<0>::fits(P.f1 - 1);
-        P.ops = family_cost(P.J.fam, fast, P.fp64 && P.f1 - 1 < FP_PMAX, P.N);
+        P.ops = family_cost(P.J.fam, fast, P.fp64 && P.f1 - 1 < FP_PMAX, P.N, P.stepped);
         if (!autotune) continue;
         const bool narrow = P.J.fam == FAM_SMARANDACHE && sm_narrow_ok(P.J.param, P.J.n1, P.f1 - 1);
-        const uint64_t key = shape_key(P.J.fam, fast, narrow, P.N);
+        const uint64_t key = shape_key(P.J.fam, fast, narrow, P.N, P.stepped);
         auto it = E->shapes.find(key);
         if (it == E->shapes.end()) {
             it = E->shapes.emplace(key, tune_shape(E, P, fast)).first;
@@ -406,10 +415,9 @@ int run_batch(ctd_hip_engine *E, const ctd_job *jobs, uint32_t njobs, uint32_t *
 
     std::vector<JobPlan> plans;
     auto t_tune = clk::now();
-    int rc = plan_jobs(E, jobs, njobs, dc.data(), df.data(), !(flags & CTD_BATCH_NO_AUTOTUNE), plans, &bs.tuned);
+    int rc = plan_jobs(E, jobs, njobs, dc.data(), df.data(), !(flags & CTD_BATCH_NO_AUTOTUNE), plans, &bs.tuned,
+                       flags);
     if (rc != CTD_OK) return rc;
-    if (flags & CTD_BATCH_NO_FP64)
-        for (auto &P : plans) P.fp64 = false;
     bs.autotune_ms = std::chrono::duration<double, std::milli>(clk::now() - t_tune).count();
 
     // tiles, longest first, each to the least-loaded stream

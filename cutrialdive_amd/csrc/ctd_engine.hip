@@ -432,6 +432,13 @@ unsigned bitlen64(uint64_t x) { return x ? 64u - (unsigned)__builtin_clzll(x) : 
 // Conditions of the smarandache 32-bit-quotient step (ctd_tf_fast.hpp):
 // m = b^digits(n) < 2^21 for every appended n < n1, bitlen(p) <= 54,
 // bitlen(m) + bitlen(p) + 1 <= 64, and n1 <= 2^32 <= p (n+1 < p).
+// The fixed-n families solve for their hits once per prime (ctd_tf_fast.hpp)
+// unless told to step every index; their index range is then never split.
+bool closed_form(uint32_t fam, uint32_t job_flags)
+{
+    return (fam == FAM_PROTH_FIXED_N || fam == FAM_RIESEL_FIXED_N) && !(job_flags & JOB_STEPPED);
+}
+
 bool sm_narrow_ok(uint64_t b, uint64_t n1, uint64_t pmax)
 {
     if (n1 > (1ull << 32)) return false;
@@ -617,8 +624,11 @@ int run_job(ctd_hip_engine *E, const ctd_job *job, const ctd_run_opts *opts, con
                 return fail(E, CTD_E_DEVICE, "prime buffer overflow in batch starting at " + std::to_string(blo));
             if (np) {
                 // index chunking so that small prime batches still fill the device
+                const uint32_t stepped = (opts && (opts->flags & CTD_RUN_STEPPED)) ? JOB_STEPPED : 0u;
                 uint64_t chunk = N, nch = 1;
-                if (opts && opts->chunk_length) {
+                if (closed_form(J.fam, stepped)) {
+                    // one solve per prime covers every index
+                } else if (opts && opts->chunk_length) {
                     chunk = opts->chunk_length;
                     nch = div_up(N, chunk);
                 } else {
@@ -639,6 +649,7 @@ int run_job(ctd_hip_engine *E, const ctd_job *job, const ctd_run_opts *opts, con
                 if (bhi - 1 < FP_PMAX && !(opts && (opts->flags & CTD_RUN_NO_FP64))) J.flags |= JOB_FP64;
                 if (opts && (opts->flags & CTD_RUN_SM_FULL_STEP)) J.flags |= JOB_SM_FULL_STEP;
                 if (bhi - 1 < (1ull << 37)) J.flags |= JOB_P37;
+                J.flags |= stepped;
                 const uint32_t block = (opts && opts->block_size > 0) ? (uint32_t)opts->block_size : 256u;
                 uint64_t items = ((np + 63) & ~63ull) * nch;
                 uint64_t grid = (opts && opts->grid_size > 0) ? (uint64_t)opts->grid_size : div_up(items, block);
@@ -1180,6 +1191,7 @@ int ctd_hip_residues(ctd_hip_engine *E, const ctd_job *job, const uint64_t *prim
         if (pmax < FP_PMAX && !(flags & CTD_RUN_NO_FP64)) J.flags |= JOB_FP64;
         if (flags & CTD_RUN_SM_FULL_STEP) J.flags |= JOB_SM_FULL_STEP;
         if (pmax < (1ull << 37)) J.flags |= JOB_P37;
+        if (flags & CTD_RUN_STEPPED) J.flags |= JOB_STEPPED;
         const uint64_t items = (nprimes + 63) & ~63ull;
         launch_tf_any(J, E->d_primes, nprimes, nullptr, fast, dim3((uint32_t)div_up(items, 256)), dim3(256),
                       E->stream);

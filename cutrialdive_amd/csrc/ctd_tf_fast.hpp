@@ -244,6 +244,42 @@ __device__ __forceinline__ bool use_fp(const JobArgs &J, uint64_t p, uint64_t st
     else return !wave_w32(p);
 }
 
+// One lane's hit at idx (lanes of a wave may be at different indices).
+__device__ __forceinline__ void record_hit_lane(const JobArgs &J, uint64_t idx, uint64_t p)
+{
+    const uint32_t pos = atomicAdd(&J.counts[idx], 1u);
+    if (pos < J.maxf) J.factors[idx * J.maxf + pos] = p;
+}
+
+// Fixed n: S(k) = k 2^n + 1 (proth) or k 2^n - 1 (riesel) vanishes mod p
+// exactly when k = -2^-n or k = 2^-n (mod p), so the hits among the indices
+// [ns, ne) are that residue class; one power per prime replaces the ne - ns
+// steps S(k+1) = S(k) + 2^n of riesel_proth_fixed_n.hpp:142-151 (the stepped
+// forms stay available as JOB_STEPPED).  2^-n is ((p + 1) / 2)^n.
+template <uint32_t FAM>
+__device__ void run_fixed_n_closed(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t ne)
+{
+    const uint64_t half = (p >> 1) + 1; // 2^-1 mod p
+    uint64_t u, r;                      // 2^-n mod p, ns mod p (both in [0, p))
+    if ((J.flags & JOB_FP64) && __ballot(p >= FP_PMAX) == 0 && __ballot(ns >= (1ull << 52)) == 0) {
+        Fmod F;
+        F.init(p);
+        u = F.canon(F.pow((double)half, J.param), p);
+        r = F.canon(F.of(ns), p);
+    } else {
+        Mont M;
+        M.init(p);
+        u = M.to_plain(M.pow(M.form_of(half), J.param));
+        r = ns % p;
+    }
+    const uint64_t ks = (FAM == FAM_PROTH_FIXED_N) ? (u ? p - u : 0) : u;
+    const uint64_t span = ne - ns;
+    for (uint64_t d = ks >= r ? ks - r : ks + (p - r); d < span; d += p) {
+        record_hit_lane(J, ns + d - J.n0, p);
+        if (span - 1 - d < p) break; // the next member is at or past ne (and d + p could wrap)
+    }
+}
+
 template <uint32_t FAM>
 struct Fast {
     __host__ __device__ static bool fits(uint64_t pmax) { return pmax < FAST_PMAX; }
@@ -257,6 +293,19 @@ struct Fast {
         auto emit = [&](uint64_t v) {
             if (J.final_res) J.final_res[pi] = v;
         };
+        if constexpr (FAM == FAM_PROTH_FIXED_N || FAM == FAM_RIESEL_FIXED_N) {
+            if (!(J.flags & JOB_STEPPED)) {
+                run_fixed_n_closed<FAM>(J, p, ns, ne);
+                if (J.final_res) {
+                    Mont M;
+                    M.init(p);
+                    GenState s;
+                    Gen<FAM>::init(s, M, J, ne - 1);
+                    emit(Gen<FAM>::plain(s, M));
+                }
+                return;
+            }
+        }
         if (use_fp<FAM>(J, p, ne - ns - 1)) {
             emit(run_fp<FAM, SHIFT>(J, p, ns, ne, ns - J.n0, ne - ns - 1));
             return;

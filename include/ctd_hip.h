@@ -84,6 +84,8 @@ typedef struct ctd_run_opts {
 #define CTD_RUN_GENERIC_ONLY 1u /* force the generic 64-bit Montgomery kernels */
 #define CTD_RUN_NO_FP64 2u      /* integer forms only (no FP64 residues below 2^48) */
 #define CTD_RUN_SM_FULL_STEP 4u /* Smarandache FP64 form: the general 7-instruction step only (no short step) */
+#define CTD_RUN_STEPPED 8u      /* fixed-n families: step every index (riesel_proth_fixed_n.hpp:142-151)
+                                   instead of solving k = -+2^-n (mod p) once per prime */
 
 typedef struct ctd_hip_engine ctd_hip_engine;
 
@@ -177,6 +179,7 @@ int ctd_hip_last_stats(const ctd_hip_engine *eng, ctd_run_stats *out);
 #define CTD_BATCH_NO_GRAPH 2u     /* issue the tiles eagerly on the streams (no graph) */
 #define CTD_BATCH_NO_AUTOTUNE 4u  /* fixed shape: 256-lane groups, one item per lane, no extra chunking */
 #define CTD_BATCH_NO_FP64 8u      /* integer residue forms only */
+#define CTD_BATCH_STEPPED 16u     /* fixed-n families step every index (as CTD_RUN_STEPPED) */
 
 typedef struct ctd_batch_opts {
     uint64_t tile_length;  /* integers of a job's prime window per tile (0 = 2^28) */

@@ -34,6 +34,8 @@ FAMILIES = [
 
 def gpu_lists(eng, seq, param, n0, n1, f0, f1, mx=1024, **kw):
     opts = None
+    if kw.get("flags") is None:
+        kw.pop("flags", None)
     if kw:
         opts = nat.CtdRunOpts(kw.get("batch_length", 0), 0, 0, kw.get("chunk_length", 0), kw.get("flags", 0))
     counts, factors, last = eng.run_job(seq, param, n0, n1, f0, f1, mx, opts=opts)
@@ -164,6 +166,41 @@ def test_unsupported_block_size_is_an_error(engine, block):
         engine.run_job(o.PROTH, 1279, 0, 80, 0, 1 << 16, 8, opts=nat.CtdRunOpts(0, 0, block, 0, 0))
 
 
+@pytest.mark.parametrize("seq", [o.PROTH_FIXED_N, o.RIESEL_FIXED_N])
+@pytest.mark.parametrize("n,k0,N,f0,f1", [
+    (1000, 0, 20000, 0, 1 << 16),                 # small primes: many hits per prime, k = 0 included
+    (7, 1, 300000, 3, 2000),                      # more indices than the primes: a class per prime
+    # the rest: a hit planted at offset 100 for a prime p0 of the window
+    (100000, None, 5000, 1 << 40, (1 << 40) + (1 << 20)),
+    (1, None, 777, (1 << 36) - 5000, (1 << 36) + 5000),
+    (64, 1 << 53, 600, 1 << 30, (1 << 30) + (1 << 16)),                   # indices past 2^52
+    (333, None, 4000, (1 << 48) - (1 << 16), (1 << 48) + (1 << 16)),      # both sides of the FP64 limit
+    (10 ** 12, None, 3000, (1 << 47) + 12345, (1 << 47) + 12345 + (1 << 14)),
+])
+def test_fixed_n_closed_form(engine, seq, n, k0, N, f0, f1):
+    """Fixed n: the hits of each prime are the k = -+2^-n (mod p) of the
+    index range, solved once per prime; they equal the stepped forms' and the
+    oracle's, also when a chunk length splits the index range."""
+    planted = k0 is None or k0 >= 1 << 50
+    if planted:
+        primes = [int(x) for x in o.sieve(f0, f1)]
+        p0 = primes[len(primes) // 2]
+        inv = pow(2, -n, p0)
+        ks = (p0 - inv) % p0 if seq == o.PROTH_FIXED_N else inv
+        if k0:  # shift the solution up by multiples of p0 to just past k0
+            ks += -(-(k0 - ks) // p0) * p0
+        k0 = ks - 100
+    k1 = k0 + N
+    r = o.trial_factor(seq, n, k0, k1, f0, f1, 64, threads=16)
+    if planted:
+        assert p0 in r[100]
+    for flags in (None, nat.CTD_RUN_STEPPED, nat.CTD_RUN_NO_FP64):
+        g, _, _ = gpu_lists(engine, seq, n, k0, k1, f0, f1, mx=64, flags=flags)
+        assert g == r, flags
+    g, _, _ = gpu_lists(engine, seq, n, k0, k1, f0, f1, mx=64, chunk_length=97)
+    assert g == r
+
+
 def test_reference_tf_cases(engine):
     # tests/tf_tests.cpp:70-109
     g, _, _ = gpu_lists(engine, o.MERSENNE, 0, 1, 101, 0, 1 << 16)
@@ -272,9 +309,11 @@ STRIDE_WINDOWS = [(1 << 36) + 24690, (1 << 37) - 2000, (1 << 40) + (1 << 39), (1
 STRIDE_OFFSETS = [0, 1, 2, 5, 11, 12, 13, 15, 16, 17, 300, 588, 589, 592, 593, 599]
 
 
-@pytest.mark.parametrize("seq", [o.PROTH, o.RIESEL, o.PROTH_FIXED_N, o.RIESEL_FIXED_N])
+@pytest.mark.parametrize("seq,flags", [(o.PROTH, None), (o.RIESEL, None), (o.PROTH_FIXED_N, None),
+                                       (o.RIESEL_FIXED_N, None), (o.PROTH_FIXED_N, nat.CTD_RUN_STEPPED),
+                                       (o.RIESEL_FIXED_N, nat.CTD_RUN_STEPPED)])
 @pytest.mark.parametrize("w", STRIDE_WINDOWS)
-def test_stride_planted_hits(engine, seq, w):
+def test_stride_planted_hits(engine, seq, flags, w):
     """At primes this large a random window holds no hits, so each job is
     built around one: p0 divides S at a chosen index.  Proth/Riesel:
     k = -+2^-n (mod p0) makes p0 | k 2^n +- 1; fixed n: the index k* = -+2^-n
@@ -295,7 +334,7 @@ def test_stride_planted_hits(engine, seq, w):
             k_hit = (p0 - inv) % p0 if seq == o.PROTH_FIXED_N else inv
             k_hit += p0 * (1 + off % 3)  # index values above p0 as well
             n0 = k_hit - off
-        g, gc, _ = gpu_lists(engine, seq, param, n0, n0 + N, f0, f1)
+        g, gc, _ = gpu_lists(engine, seq, param, n0, n0 + N, f0, f1, flags=flags)
         assert p0 in g[off], (seq, w, off)
         assert_same(g, gc, seq, param, n0, n0 + N, f0, f1)
 
@@ -357,9 +396,10 @@ def _canon(v, p):
     return v if v <= (p - 1) // 2 else v - p
 
 
-@pytest.mark.parametrize("seq", [o.PROTH, o.PROTH_FIXED_N])
+@pytest.mark.parametrize("seq,flags", [(o.PROTH, None), (o.PROTH_FIXED_N, nat.CTD_RUN_STEPPED),
+                                       (o.PROTH_FIXED_N, None)])
 @pytest.mark.parametrize("j", [0, 5, 63])
-def test_word_filter_collision_is_not_a_hit(engine, seq, j):
+def test_word_filter_collision_is_not_a_hit(engine, seq, flags, j):
     """The 64-index stride forms compare 32-bit words (the low bits of the
     residue); a state that differs from a target by exactly 2^32 matches its
     word without being a hit.  The job is built so that the first block of a
@@ -387,6 +427,6 @@ def test_word_filter_collision_is_not_a_hit(engine, seq, j):
         n0 = ((x - 1) * pow(c, -1, p0)) % p0
         hit_index = n0 + j + 1
         assert (hit_index * c + 1) % p0 != 0
-    g, gc, _ = gpu_lists(engine, seq, param, n0, n0 + N, f0, f1)
+    g, gc, _ = gpu_lists(engine, seq, param, n0, n0 + N, f0, f1, flags=flags)
     assert p0 not in g[hit_index - n0]
     assert_same(g, gc, seq, param, n0, n0 + N, f0, f1)

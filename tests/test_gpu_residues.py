@@ -60,7 +60,10 @@ def test_final_residues(engine, seq, param, win):
     primes = random_primes(lo, hi, 256, seed=seq * 97 + win)
     n0, n1 = (5, 2100) if seq != o.PEANO else (10 ** 12, 10 ** 12 + 500)
     want = o.residues_last(seq, param, n0, n1 - n0, primes)
-    for flags in (0, nat.CTD_RUN_NO_FP64, nat.CTD_RUN_GENERIC_ONLY):
+    flag_sets = [0, nat.CTD_RUN_NO_FP64, nat.CTD_RUN_GENERIC_ONLY]
+    if seq in (o.PROTH_FIXED_N, o.RIESEL_FIXED_N):  # the stepped stride loops as well
+        flag_sets += [nat.CTD_RUN_STEPPED, nat.CTD_RUN_STEPPED | nat.CTD_RUN_NO_FP64]
+    for flags in flag_sets:
         got = engine.residues(seq, param, n0, n1, primes, flags)
         bad = np.nonzero(got != want)[0]
         assert len(bad) == 0, (flags, [(primes[i], int(got[i]), int(want[i])) for i in bad[:5]])
