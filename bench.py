@@ -585,7 +585,7 @@ def run_mixed(args, torch, ctd, nat):
             "graph_launches_per_step": bat["graphs"] // max(args.steps, 1),
             "graph_build_ms_per_step": bat["build_ms"] / max(args.steps, 1),
             "autotune": not args.no_autotune,
-            "shapes": {k: {"block": b, "waves_per_cu": w, "chunk_split": s} for k, (b, w, s) in sorted(shapes.items())},
+            "shapes": {k: {"block": b, "primes_per_lane": w, "chunk_split": s} for k, (b, w, s) in sorted(shapes.items())},
         },
         "plain": result_plain,
         "batched_vs_plain": (value / result_plain["value"]) if result_plain else None,

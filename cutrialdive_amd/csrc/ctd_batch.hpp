@@ -14,15 +14,18 @@
 // hipGraph and launched with one hipGraphLaunch per 1024 tiles.
 //
 // Autotuner: for each (family, kernel kind, index-count class) the first
-// batch that needs it times a probe tile under a small set of launch shapes
-// (workgroup size, waves per CU of a grid-stride grid, index-chunk split) and
-// keeps the fastest.  Shapes change the schedule only: every shape computes
-// the same residues and records the same hits.
+// batch that needs it times a probe tile under every launch shape of a small
+// grid -- workgroup size (64, 256 lanes), primes per lane (1, 2, 4: each lane
+// takes that many work items in turn) and index-chunk split (1, 2, 4) --
+// with one warm-up and three timed runs each, and keeps the shape of the
+// lowest median.  Shapes change the schedule only: every shape computes the
+// same residues and records the same hits.
 
 constexpr uint32_t BATCH_GRAPH_TILES = 1024;
 constexpr uint64_t BATCH_TILE_DEFAULT = 1ull << 28;
 constexpr uint64_t TUNE_WINDOW = 1ull << 25;
-constexpr uint64_t TUNE_MAX_N = 2048;
+constexpr uint64_t TUNE_MAX_N = 16384;
+constexpr int TUNE_REPS = 3;
 
 // Montgomery-Vaughan bound pi(x + y) - pi(x) <= 2y / log y, plus slack.
 uint64_t prime_cap(uint64_t len)
@@ -155,9 +158,9 @@ JobArgs tile_args(const ctd_hip_engine *E, const JobPlan &P, uint64_t est_np, ui
 
 dim3 tile_grid(const ctd_hip_engine *E, const LaunchShape &sh, uint64_t est_np, uint32_t nchunks)
 {
+    (void)E;
     const uint64_t items = (((uint64_t)(1.1 * (double)est_np) + 64 + 63) & ~63ull) * nchunks;
-    uint64_t g = div_up(items, sh.block);
-    if (sh.wpc) g = std::min<uint64_t>(g, div_up((uint64_t)E->cu * sh.wpc * 64, sh.block));
+    const uint64_t g = div_up(items, (uint64_t)sh.block * sh.ppl);
     return dim3((uint32_t)std::max<uint64_t>(std::min<uint64_t>(g, 1ull << 30), 1));
 }
 
@@ -261,27 +264,29 @@ LaunchShape tune_shape(ctd_hip_engine *E, const JobPlan &P, bool fast)
     Q.J.counts = E->d_scr_counts;
     Q.J.factors = E->d_scr_factors;
     const uint32_t blocks[] = {64u, 256u};
-    const uint32_t wpcs[] = {0u, 16u, 32u};
-    const uint32_t splits[] = {1u, 2u};
+    const uint32_t ppls[] = {1u, 2u, 4u};
+    const uint32_t splits[] = {1u, 2u, 4u};
     float best_ms = 1e30f;
     for (uint32_t b : blocks)
-        for (uint32_t w : wpcs)
+        for (uint32_t pl : ppls)
             for (uint32_t sp : splits) {
-                if (sp > 1 && Q.N < 2048) continue;
-                Q.shape = LaunchShape{b, w, sp};
+                if (sp > 1 && (Q.N < 1024 * sp || closed_form(Q.J.fam, Q.stepped))) continue;
+                Q.shape = LaunchShape{b, pl, sp};
                 JobArgs J = tile_args(E, Q, np, hi - 1);
                 dim3 grid = tile_grid(E, Q.shape, np, J.nchunks);
-                float ms = 0.f;
-                for (int rep = 0; rep < 2; ++rep) { // the first run warms the code object
+                float ms[TUNE_REPS + 1];
+                for (int rep = 0; rep <= TUNE_REPS; ++rep) { // the first run warms the code object
                     HIPCHK(hipEventRecord(E->ev0, st));
                     launch_tf_any(J, S.primes, np, nullptr, fast, grid, dim3(b), st);
                     HIPCHK(hipGetLastError());
                     HIPCHK(hipEventRecord(E->ev1, st));
                     HIPCHK(hipStreamSynchronize(st));
-                    HIPCHK(hipEventElapsedTime(&ms, E->ev0, E->ev1));
+                    HIPCHK(hipEventElapsedTime(&ms[rep], E->ev0, E->ev1));
                 }
-                if (ms < best_ms) {
-                    best_ms = ms;
+                std::sort(ms + 1, ms + TUNE_REPS + 1);
+                const float med = ms[1 + TUNE_REPS / 2];
+                if (med < best_ms) {
+                    best_ms = med;
                     best = Q.shape;
                 }
             }

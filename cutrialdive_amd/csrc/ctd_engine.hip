@@ -139,11 +139,13 @@ __global__ void tile_done_kernel(const unsigned long long *sc, uint64_t cap, uns
     }
 }
 
-// Launch shape of a residue kernel: lanes per workgroup, waves per CU of a
-// grid-stride grid (0: one work item per lane) and the extra index-chunk split.
+// Launch shape of a residue kernel: lanes per workgroup, work items (primes)
+// each lane takes in turn (the grid holds items / (block * ppl) lanes; the
+// sequential counterpart of the reference's B primes per thread,
+// trial_factoring.cuh:472-481) and the extra index-chunk split.
 struct LaunchShape {
     uint32_t block = 256;
-    uint32_t wpc = 0;
+    uint32_t ppl = 1;
     uint32_t split = 1;
 };
 
@@ -896,7 +898,7 @@ int ctd_hip_run_batch(ctd_hip_engine *E, const ctd_job *jobs, uint32_t njobs, ui
     }
 }
 
-int ctd_hip_batch_shape(ctd_hip_engine *E, const ctd_job *job, uint32_t *block, uint32_t *waves_per_cu,
+int ctd_hip_batch_shape(ctd_hip_engine *E, const ctd_job *job, uint32_t *block, uint32_t *primes_per_lane,
                         uint32_t *chunk_split)
 {
     if (!E || !job) return CTD_E_INVALID;
@@ -912,7 +914,7 @@ int ctd_hip_batch_shape(ctd_hip_engine *E, const ctd_job *job, uint32_t *block, 
         int rc = plan_jobs(E, job, 1, &c, &f, true, plans, &tuned);
         if (rc != CTD_OK) return rc;
         if (block) *block = plans[0].shape.block;
-        if (waves_per_cu) *waves_per_cu = plans[0].shape.wpc;
+        if (primes_per_lane) *primes_per_lane = plans[0].shape.ppl;
         if (chunk_split) *chunk_split = plans[0].shape.split;
         return CTD_OK;
     } catch (const HipError &h) {

@@ -300,7 +300,7 @@ class Engine:
         return out, st
 
     def batch_shape(self, job: nat.CtdJob):
-        """(block, waves per CU, chunk split) the batch autotuner picks for the job."""
+        """(block, primes per lane, chunk split) the batch autotuner picks for the job."""
         b, w, s = nat.u32(), nat.u32(), nat.u32()
         self._check(self.lib.ctd_hip_batch_shape(self.h, ctypes.byref(job), ctypes.byref(b), ctypes.byref(w),
                                                  ctypes.byref(s)))

@@ -207,10 +207,12 @@ int ctd_hip_run_batch(ctd_hip_engine *eng, const ctd_job *jobs, uint32_t njobs,
                       const ctd_batch_opts *opts, ctd_batch_stats *stats, void *stream);
 
 /* Launch shape chosen for a job by the batch autotuner (probing it if needed):
- * block (lanes per workgroup), waves per CU of the grid (0 = one item per
- * lane) and the index-chunk split.  Test and report hook. */
+ * block (lanes per workgroup: 64 or 256), primes per lane (work items each
+ * lane takes in turn: 1, 2 or 4; the reference runs B = 1..3 primes per
+ * thread, trial_factoring.cuh:472-481) and the index-chunk split (1, 2 or 4).
+ * Test and report hook. */
 int ctd_hip_batch_shape(ctd_hip_engine *eng, const ctd_job *job, uint32_t *block,
-                        uint32_t *waves_per_cu, uint32_t *chunk_split);
+                        uint32_t *primes_per_lane, uint32_t *chunk_split);
 
 /* ---- Serving form -------------------------------------------------------
  * Continuous batching: jobs are submitted while others run.  A scheduler

@@ -139,8 +139,8 @@ def test_edge_jobs(engine):
 
 def test_batch_shape_is_valid(engine):
     s = workloads.mixed_jobs(1)[0]
-    b, w, sp = engine.batch_shape(job_of(s))
-    assert b in (64, 256) and w in (0, 16, 32) and sp in (1, 2)
+    b, ppl, sp = engine.batch_shape(job_of(s))
+    assert b in (64, 256) and ppl in (1, 2, 4) and sp in (1, 2, 4)
 
 
 def test_invalid_job_is_rejected(engine):

@@ -166,6 +166,27 @@ def test_unsupported_block_size_is_an_error(engine, block):
         engine.run_job(o.PROTH, 1279, 0, 80, 0, 1 << 16, 8, opts=nat.CtdRunOpts(0, 0, block, 0, 0))
 
 
+# Primes whose order of 2 is below 64 repeat stride targets: p | 2^59 - 1
+# (order 59) and p | 2^47 - 1 (order 47), both below 2^48.
+SMALL_ORDER_PRIMES = [(3203431780337, 59), (13264529, 47)]
+
+
+@pytest.mark.parametrize("p,order", SMALL_ORDER_PRIMES)
+@pytest.mark.parametrize("flags", [None, nat.CTD_RUN_NO_FP64])
+def test_doubling_small_order_primes(engine, p, order, flags):
+    """Mersenne: p | 2^n - 1 exactly when order | n; Proth k = p - 1 (k 2^n + 1
+    = 1 - 2^n mod p) likewise.  The 64 stride targets repeat (j and j + order
+    give the same value) and every multiple of the order must be found, in
+    64-index blocks and in the tail."""
+    f0, f1 = p - 2000, p + 2000
+    for seq, k in ((o.MERSENNE, 0), (o.PROTH, p - 1), (o.RIESEL, 1)):
+        n0, n1 = 3, 3 + 64 * 37 + 29
+        r = o.trial_factor(seq, k, n0, n1, f0, f1, 64)
+        assert sum(1 for i, fs in enumerate(r) if p in fs) == len([n for n in range(n0, n1) if n % order == 0])
+        g, _, _ = gpu_lists(engine, seq, k, n0, n1, f0, f1, mx=64, flags=flags)
+        assert g == r, (seq, flags)
+
+
 @pytest.mark.parametrize("seq", [o.PROTH_FIXED_N, o.RIESEL_FIXED_N])
 @pytest.mark.parametrize("n,k0,N,f0,f1", [
     (1000, 0, 20000, 0, 1 << 16),                 # small primes: many hits per prime, k = 0 included
@@ -306,7 +327,7 @@ STRIDE_WINDOWS = [(1 << 36) + 24690, (1 << 37) - 2000, (1 << 40) + (1 << 39), (1
 # Offsets of the planted hit from n0: the check before the loop, every
 # position of a 12- and a 16-index block, and the single-step tails (N = 600:
 # 16-index blocks cover offsets 1..592, 12-index blocks 1..588).
-STRIDE_OFFSETS = [0, 1, 2, 5, 11, 12, 13, 15, 16, 17, 300, 588, 589, 592, 593, 599]
+STRIDE_OFFSETS = [0, 1, 2, 5, 11, 12, 13, 15, 16, 17, 63, 64, 65, 300, 575, 576, 577, 588, 589, 592, 593, 599]
 
 
 @pytest.mark.parametrize("seq,flags", [(o.PROTH, None), (o.RIESEL, None), (o.PROTH_FIXED_N, None),
