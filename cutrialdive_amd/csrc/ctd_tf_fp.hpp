@@ -297,6 +297,62 @@ __device__ __forceinline__ uint32_t low_word(double v)
     return (uint32_t)__double_as_longlong(v + Fmod::MAGIC);
 }
 
+// Compares against 16 targets that cost the scalar unit 3 instructions instead
+// of 16: each v_cmpx_ne removes the lanes whose state equals its target from
+// EXEC, and the lanes that left (saved EXEC xor EXEC) are the wave's matches;
+// EXEC is restored inside the same statement.  (A compare per target into a
+// lane mask costs the scalar unit an OR per compare, and the scalar unit
+// issues for all four SIMDs of a CU.)  On gfx950 the 32-bit form writes VCC
+// as well, and a VALU instruction reads the EXEC a v_cmpx wrote without wait
+// states.
+__device__ __forceinline__ uint64_t words_match16(uint32_t w, const uint64_t *Tw)
+{
+    uint64_t save, gone;
+    asm volatile("s_mov_b64 %[s], exec\n\t"
+                 "v_cmpx_ne_u32 vcc, %[w], %[t0]\n\tv_cmpx_ne_u32 vcc, %[w], %[t1]\n\t"
+                 "v_cmpx_ne_u32 vcc, %[w], %[t2]\n\tv_cmpx_ne_u32 vcc, %[w], %[t3]\n\t"
+                 "v_cmpx_ne_u32 vcc, %[w], %[t4]\n\tv_cmpx_ne_u32 vcc, %[w], %[t5]\n\t"
+                 "v_cmpx_ne_u32 vcc, %[w], %[t6]\n\tv_cmpx_ne_u32 vcc, %[w], %[t7]\n\t"
+                 "v_cmpx_ne_u32 vcc, %[w], %[t8]\n\tv_cmpx_ne_u32 vcc, %[w], %[t9]\n\t"
+                 "v_cmpx_ne_u32 vcc, %[w], %[ta]\n\tv_cmpx_ne_u32 vcc, %[w], %[tb]\n\t"
+                 "v_cmpx_ne_u32 vcc, %[w], %[tc]\n\tv_cmpx_ne_u32 vcc, %[w], %[td]\n\t"
+                 "v_cmpx_ne_u32 vcc, %[w], %[te]\n\tv_cmpx_ne_u32 vcc, %[w], %[tf]\n\t"
+                 "s_xor_b64 %[g], %[s], exec\n\t"
+                 "s_mov_b64 exec, %[s]"
+                 : [s] "=&s"(save), [g] "=&s"(gone)
+                 : [w] "v"(w), [t0] "v"((uint32_t)Tw[0]), [t1] "v"((uint32_t)(Tw[0] >> 32)),
+                   [t2] "v"((uint32_t)Tw[1]), [t3] "v"((uint32_t)(Tw[1] >> 32)), [t4] "v"((uint32_t)Tw[2]),
+                   [t5] "v"((uint32_t)(Tw[2] >> 32)), [t6] "v"((uint32_t)Tw[3]), [t7] "v"((uint32_t)(Tw[3] >> 32)),
+                   [t8] "v"((uint32_t)Tw[4]), [t9] "v"((uint32_t)(Tw[4] >> 32)), [ta] "v"((uint32_t)Tw[5]),
+                   [tb] "v"((uint32_t)(Tw[5] >> 32)), [tc] "v"((uint32_t)Tw[6]), [td] "v"((uint32_t)(Tw[6] >> 32)),
+                   [te] "v"((uint32_t)Tw[7]), [tf] "v"((uint32_t)(Tw[7] >> 32))
+                 : "vcc", "scc");
+    return gone;
+}
+
+// The same for 16 exact FP64 targets.
+__device__ __forceinline__ uint64_t doubles_match16(double x, const double *T)
+{
+    uint64_t save, gone;
+    asm volatile("s_mov_b64 %[s], exec\n\t"
+                 "v_cmpx_neq_f64 vcc, %[x], %[t0]\n\tv_cmpx_neq_f64 vcc, %[x], %[t1]\n\t"
+                 "v_cmpx_neq_f64 vcc, %[x], %[t2]\n\tv_cmpx_neq_f64 vcc, %[x], %[t3]\n\t"
+                 "v_cmpx_neq_f64 vcc, %[x], %[t4]\n\tv_cmpx_neq_f64 vcc, %[x], %[t5]\n\t"
+                 "v_cmpx_neq_f64 vcc, %[x], %[t6]\n\tv_cmpx_neq_f64 vcc, %[x], %[t7]\n\t"
+                 "v_cmpx_neq_f64 vcc, %[x], %[t8]\n\tv_cmpx_neq_f64 vcc, %[x], %[t9]\n\t"
+                 "v_cmpx_neq_f64 vcc, %[x], %[ta]\n\tv_cmpx_neq_f64 vcc, %[x], %[tb]\n\t"
+                 "v_cmpx_neq_f64 vcc, %[x], %[tc]\n\tv_cmpx_neq_f64 vcc, %[x], %[td]\n\t"
+                 "v_cmpx_neq_f64 vcc, %[x], %[te]\n\tv_cmpx_neq_f64 vcc, %[x], %[tf]\n\t"
+                 "s_xor_b64 %[g], %[s], exec\n\t"
+                 "s_mov_b64 exec, %[s]"
+                 : [s] "=&s"(save), [g] "=&s"(gone)
+                 : [x] "v"(x), [t0] "v"(T[0]), [t1] "v"(T[1]), [t2] "v"(T[2]), [t3] "v"(T[3]), [t4] "v"(T[4]),
+                   [t5] "v"(T[5]), [t6] "v"(T[6]), [t7] "v"(T[7]), [t8] "v"(T[8]), [t9] "v"(T[9]), [ta] "v"(T[10]),
+                   [tb] "v"(T[11]), [tc] "v"(T[12]), [td] "v"(T[13]), [te] "v"(T[14]), [tf] "v"(T[15])
+                 : "vcc", "scc");
+    return gone;
+}
+
 // nb blocks of NS doubling strides of S indices for p < 2^(53-S) (x advances
 // 2^S per stride by an exact shift, T[j] = t 2^-(j+1)).  The scalar unit
 // issues the OR of every compare's wave mask and the loop control for the
@@ -316,8 +372,8 @@ __device__ __forceinline__ double doubling_blocks(const JobArgs &J, uint64_t p, 
         uint64_t any = 0;
         _Pragma("unroll")
         for (int h = 0; h < NS; ++h) {
-            _Pragma("unroll")
-            for (int j = 0; j < S; ++j) any |= __ballot(x == T[j]);
+            static_assert(S == 16, "one 16-target compare block per stride");
+            any |= doubles_match16(x, T);
             x = fma(x, c, -fma(fma(x, ip, Fmod::MAGIC), P, -pM));
         }
         if (any) {
@@ -352,10 +408,7 @@ __device__ __forceinline__ double doubling_words(const JobArgs &J, uint64_t p, u
         const uint32_t xw = low_word(x);
         uint64_t any = 0;
         _Pragma("unroll")
-        for (int j = 0; j < B / 2; ++j) {
-            any |= __ballot(xw == (uint32_t)Tw[j]);
-            any |= __ballot(xw == (uint32_t)(Tw[j] >> 32));
-        }
+        for (int q = 0; q < B / 16; ++q) any |= words_match16(xw, Tw + 8 * q);
         if (any) {
             const uint32_t lane = __lane_id();
             const uint64_t at = hit_base(idx, i * (uint32_t)B);
@@ -384,10 +437,7 @@ __device__ __forceinline__ double fixedn_blocks(const JobArgs &J, uint64_t p, ui
         const uint32_t xw = low_word(x);
         uint64_t any = 0;
         _Pragma("unroll")
-        for (int j = 0; j < B / 2; ++j) {
-            any |= __ballot(xw == (uint32_t)Tw[j]);
-            any |= __ballot(xw == (uint32_t)(Tw[j] >> 32));
-        }
+        for (int q = 0; q < B / 16; ++q) any |= words_match16(xw, Tw + 8 * q);
         if (any) {
             const uint32_t lane = __lane_id();
             const uint64_t at = hit_base(idx, i * (uint32_t)B);
@@ -546,10 +596,7 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
                 const uint32_t xw = low_word(x.v);
                 uint64_t any = 0;
                 _Pragma("unroll")
-                for (int j = 0; j < THETA_STRIDE / 2; ++j) {
-                    any |= __ballot(xw == (uint32_t)Tw[j]);
-                    any |= __ballot(xw == (uint32_t)(Tw[j] >> 32));
-                }
+                for (int q = 0; q < THETA_STRIDE / 16; ++q) any |= words_match16(xw, Tw + 8 * q);
                 if (any) {
                     // replay the block: x theta^(j+1) == eps on both coordinates
                     // (x is made opaque so that the loop's masks are not kept
