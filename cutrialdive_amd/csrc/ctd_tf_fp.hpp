@@ -652,12 +652,15 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
         trib_triple_fp(F, ns, a, b, c);
         record_hits(J, idx, p, a == 0.0);
         if (__ballot(p >= TRIB_LAZY_PMAX) == 0) {
-            // Lazy form, 8 indices per block (39 instructions): from the
-            // reduced state (T(n), T(n+1), T(n+2)), |.| <= 0.75 p, the sums
-            // x[j] = T(n+3+j) are formed unreduced (|x[7]| <= 144.75 p < 2^51.2
-            // for p < 2^44, so every sum is exact).  Each x[j] is tested
+            // Lazy form, 8 indices per block (36 instructions): from the
+            // reduced state (T(n), T(n+1), T(n+2)), |.| <= 0.75 p, the terms
+            // x[j] = T(n+3+j) are formed unreduced: T(n+3) = a + b + c, then
+            // T(k+4) = 2 T(k+3) - T(k) (subtract T(k+3) = T(k+2) + T(k+1) + T(k)
+            // from T(k+4) = T(k+3) + T(k+2) + T(k+1)), one exact FMA per term;
+            // |x[j]| <= 2 |x[j-1]| + |x[j-4]| gives |x[7]| <= 456.75 p < 2^53
+            // for p < 2^44, so every term is exact.  Each x[j] is tested
             // without reducing it: with q the biased rounded quotient,
-            // w = fma(q, -p, M p) = -round(x/p) p exactly (|w| < 2^52), and
+            // w = fma(q, -p, M p) = -round(x/p) p exactly (|w| < 2^53), and
             // x = 0 mod p <=> x == -w.  Only the new state x[5..7] is reduced
             // (x + w, |.| <= p/2 + tiny).  T(n+1) and T(n+2) are reduced already
             // and compare with 0.
@@ -666,18 +669,14 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
                 {
                     double x[8];
                     double w[8];
-                    double s = b + c;
-                    x[0] = a + s;
-                    x[1] = s + x[0];
-                    s = x[0] + x[1];
-                    x[2] = c + s;
-                    x[3] = s + x[2];
-                    s = x[2] + x[3];
-                    x[4] = x[1] + s;
-                    x[5] = s + x[4];
-                    s = x[4] + x[5];
-                    x[6] = x[3] + s;
-                    x[7] = s + x[6];
+                    x[0] = a + (b + c);
+                    x[1] = fma(2.0, x[0], -a);
+                    x[2] = fma(2.0, x[1], -b);
+                    x[3] = fma(2.0, x[2], -c);
+                    x[4] = fma(2.0, x[3], -x[0]);
+                    x[5] = fma(2.0, x[4], -x[1]);
+                    x[6] = fma(2.0, x[5], -x[2]);
+                    x[7] = fma(2.0, x[6], -x[3]);
                     mk[0] = __ballot(b == 0.0);
                     mk[1] = __ballot(c == 0.0);
                     _Pragma("unroll")
