@@ -39,10 +39,12 @@
 // Per index this costs (VALU instructions, compare included):
 //
 //   doubling x = 2x           1.19 (p < 2^37), 1.16 (integer W64 form: 7)
-//   fixed n  x = x + c        1.08 (6)
+//   fixed n  x = x + c        1.08 (6) when stepped; by default the closed
+//                             form of ctd_tf_fast.hpp solves once per prime
 //   fibonacci / lucas         1.53 theta stride form, 5 pair form (6)
-//   tribonacci                4.875 (p < 2^44), 5.5 (11.25)
-//   smarandache r m + n + 1   7   (12.25), for p > 8 n1; 5 when p (0.75 m + 1) < 2^53
+//   tribonacci                4.5 (p < 2^44), 5.5 (11.25)
+//   smarandache r m + n + 1   7   (12.25), for p > 8 n1; 5 when p (0.75 m + 1) < 2^53,
+//                             6 when 0.75 p o < 2^53 (o the odd part of m)
 //
 // and the per-prime set-up (powers, doubling formulas, Smarandache jump) runs
 // on 6-instruction multiplies instead of ~40-instruction Montgomery products.
@@ -728,6 +730,13 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
         // rounding constant, M p exact for p < 2^50), r m + w = r m - k p is
         // exact, and r' = (r m - k p) + c takes 4 instructions and the compare
         // (the same bound |r'| <= 0.75 p holds).
+        //
+        // Exact-product form, for a segment where every lane has 0.75 p o <
+        // 2^53 with o the odd part of m (m = o 2^a): |r o| < 2^53, so r o is
+        // exact and so is r m = (r o) 2^a (a power-of-two scaling).  Then
+        // fma(-k, p, r m) has the exact value r m - k p, |.| <= (1/2 + 2^-31) p
+        // < 2^53, so it is exact, and r' = (r m - k p) + c takes 5 instructions
+        // and the compare.  Power-of-two bases (o = 1) always qualify.
         const uint64_t bb = J.param;
         double r = sm_value_fp(F, bb, ns);
         record_hits(J, idx, p, r == 0.0);
@@ -737,8 +746,12 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
         uint32_t d;
         uint64_t nxt;
         digits_pow(bb, ns + 1, d, nxt);
-        double m = 1.0;
-        for (uint32_t i = 0; i < d; ++i) m *= (double)bb;
+        const double ob = (double)(bb >> __builtin_ctzll(bb)); // odd part of the base
+        double m = 1.0, om = 1.0;                                // m = bb^d, om its odd part
+        for (uint32_t i = 0; i < d; ++i) {
+            m *= (double)bb;
+            om *= ob;
+        }
         double mp = m * F.invp;
         const double pM = F.p * Fmod::MAGIC;
         uint64_t n = ns; // index of the current state
@@ -753,8 +766,11 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
             const uint64_t delta = 1ull << (52 - e);
             uint64_t cb = __double_as_longlong((double)c0);
             // one wave-uniform choice per segment (m is fixed inside it)
-            const bool short_form =
-                !(J.flags & JOB_SM_FULL_STEP) && __ballot(!(F.p * (0.75 * m + 1.0) < SM_SHORT_LIMIT)) == 0;
+            const bool free_form = !(J.flags & JOB_SM_FULL_STEP);
+            const bool short_form = free_form && !(J.flags & JOB_SM_NO_SHORT) &&
+                                    __ballot(!(F.p * (0.75 * m + 1.0) < SM_SHORT_LIMIT)) == 0;
+            const bool exact_form =
+                !short_form && free_form && __ballot(!(F.p * (0.75 * om) < SM_SHORT_LIMIT)) == 0;
             auto run = [&](auto step) {
                 uint32_t kk = 0;
                 for (; kk + UNROLL <= cnt; kk += UNROLL) {
@@ -785,6 +801,13 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
                     r = fma(r, m, fma(k1, -F.p, pM)) + __longlong_as_double(cb);
                     cb += delta;
                 });
+            } else if (exact_form) {
+                run([&](void) {
+                    const double rm = r * m;
+                    const double k = fma(r, mp, Fmod::MAGIC) - Fmod::MAGIC;
+                    r = fma(-k, F.p, rm) + __longlong_as_double(cb);
+                    cb += delta;
+                });
             } else {
                 run([&](void) {
                     const double k = fma(r, mp, Fmod::MAGIC) - Fmod::MAGIC;
@@ -796,6 +819,7 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
             n = seg_last + 1;
             if (n + 1 == nxt) { // the next appended number has one more digit
                 m *= (double)bb;
+                om *= ob;
                 mp = m * F.invp;
                 nxt = sat_mul(nxt, bb);
             }

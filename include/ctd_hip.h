@@ -86,6 +86,8 @@ typedef struct ctd_run_opts {
 #define CTD_RUN_SM_FULL_STEP 4u /* Smarandache FP64 form: the general 7-instruction step only (no short step) */
 #define CTD_RUN_STEPPED 8u      /* fixed-n families: step every index (riesel_proth_fixed_n.hpp:142-151)
                                    instead of solving k = -+2^-n (mod p) once per prime */
+#define CTD_RUN_SM_NO_SHORT 16u /* Smarandache FP64 form: no short step (the 6-instruction exact-product
+                                   step where r b^digits is exact, else the general step); test hook */
 
 typedef struct ctd_hip_engine ctd_hip_engine;
 

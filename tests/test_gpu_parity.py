@@ -400,14 +400,15 @@ def test_theta_form_windows(engine, seq, n0, n1, f0, f1, chunk):
 @pytest.mark.parametrize("base", [10, 2, 16])
 def test_smarandache_step_forms_with_hits(engine, base):
     # primes in [2^20, 2^21) are above 8 n1, so the FP64 form runs; p m is far
-    # below 2^53, so it takes the short step by default, and
-    # CTD_RUN_SM_FULL_STEP runs the general step on the same primes.  About
-    # 250 hits per base.
+    # below 2^53, so it takes the short step by default,
+    # CTD_RUN_SM_NO_SHORT runs the exact-product step (p o is far below 2^53
+    # too) and CTD_RUN_SM_FULL_STEP the general step on the same primes.
+    # About 250 hits per base.
     n0, n1 = 1, 5000
     f0, f1 = 1 << 20, 1 << 21
     r = o.trial_factor(o.SMARANDACHE, base, n0, n1, f0, f1, threads=16)
     assert sum(len(x) for x in r) > 100
-    for flags in (0, nat.CTD_RUN_SM_FULL_STEP):
+    for flags in (0, nat.CTD_RUN_SM_NO_SHORT, nat.CTD_RUN_SM_FULL_STEP):
         g, _, _ = gpu_lists(engine, o.SMARANDACHE, base, n0, n1, f0, f1, flags=flags)
         assert g == r, flags
 

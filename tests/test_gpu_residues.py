@@ -140,6 +140,61 @@ def test_smarandache_short_step_limit(engine, base):
             assert len(bad) == 0, (m, flags, [(primes[i], int(got[i]), int(want[i])) for i in bad[:5]])
 
 
+def _odd_part(m):
+    while m % 2 == 0:
+        m //= 2
+    return m
+
+
+@pytest.mark.parametrize("base", [10, 12, 6, 14, 2])
+def test_smarandache_exact_product_limit(engine, base):
+    # The exact-product Smarandache step needs 0.75 p o < 2^53 for the odd
+    # part o of the digit multiplier m (then r m is exact); a wave takes it
+    # when all its lanes qualify and the short step does not.  For each
+    # multiplier whose limit is below 2^48: a wave of primes just below the
+    # limit, a wave just above it, and waves that interleave primes below it
+    # with primes at 2-4 times it.  Power-of-two bases (o = 1) qualify at
+    # every prime below 2^48: waves spread over [2^40, 2^48).  Final residues
+    # with every step allowed, without the short step (CTD_RUN_SM_NO_SHORT)
+    # and with the general step only.
+    n0, n1 = 5, 2100
+    mults = sorted({_digits_multiplier(base, n) for n in range(n0, n1)})
+    fp_max = (1 << 48) - 1
+    rng = random.Random(base + 100)
+    sets = []
+    for m in mults:
+        lim = int((2 ** 53 - 2 ** 30) / (0.75 * _odd_part(m)))
+        if lim >= fp_max:
+            continue
+        below, above, far = [], [], []
+        p = lim
+        while len(below) < 96:
+            p = int(sympy.prevprime(p))
+            below.append(p)
+        p = lim
+        while len(above) < 64 and p < fp_max:
+            p = int(sympy.nextprime(p))
+            if p < fp_max:
+                above.append(p)
+        far_hi = min(4 * lim, fp_max - 1000)
+        while 2 * lim < far_hi and len(far) < 32:
+            q = int(sympy.nextprime(rng.randrange(2 * lim, far_hi)))
+            if q < fp_max:
+                far.append(q)
+        mixed = [x for pair in zip(below[64:], far) for x in pair] if far else below[64:]
+        sets.append((m, below[:64] + above + mixed))
+    if _odd_part(base) == 1:
+        spread = sorted(int(sympy.nextprime(rng.randrange(1 << 40, fp_max - 1000))) for _ in range(192))
+        sets.append((0, spread))
+    assert sets
+    for m, primes in sets:
+        want = o.residues_last(o.SMARANDACHE, base, n0, n1 - n0, primes)
+        for flags in (0, nat.CTD_RUN_SM_NO_SHORT, nat.CTD_RUN_SM_FULL_STEP):
+            got = engine.residues(o.SMARANDACHE, base, n0, n1, primes, flags)
+            bad = np.nonzero(got != want)[0]
+            assert len(bad) == 0, (m, flags, [(primes[i], int(got[i]), int(want[i])) for i in bad[:5]])
+
+
 def test_tribonacci_lazy_form_limit(engine):
     # The 8-index lazy Tribonacci form runs on waves whose primes are all
     # below 2^44; a wave of primes just below, one just above, and waves that
