@@ -224,11 +224,6 @@ __device__ __forceinline__ Rphi rphi_mul_theta(const Fmod &F, Rphi x)
     const double s = x.u + x.v;
     return {F.red(-s), F.red(-s - x.v)};
 }
-// x theta^-1 = (v - 2u) + (u - v) phi   (theta^-1 = -2 + phi)
-__device__ __forceinline__ Rphi rphi_mul_theta_inv(const Fmod &F, Rphi x)
-{
-    return {F.red(x.v - 2.0 * x.u), F.red(x.u - x.v)};
-}
 // x y = (xu yu + xv yv) + (xu yv + xv yu + xv yv) phi
 __device__ __forceinline__ Rphi rphi_mul(const Fmod &F, Rphi x, Rphi y)
 {
@@ -575,15 +570,21 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
             // Equal v give equal words; a word match that is not a hit (values
             // 2^32 apart) is rejected by the exact replay.  Two words share a
             // 64-bit register pair, so the targets take THETA_STRIDE registers.
+            // The v coordinates of eps theta^-j follow v_(j+1) = -3 v_j - v_(j-1)
+            // from v_0 = 0, v_1 = eps: theta^-1 = phi - 2 is a root of
+            // y^2 + 3y + 1 (theta + theta^-1 = -(phi^2 + psi^2) = -L(2) = -3),
+            // so every coordinate of eps theta^-j obeys that recurrence.  The
+            // sum stays below 2p + 1, so red gives the same (symmetric)
+            // representative as the ring product does for x.
             uint64_t Tw[THETA_STRIDE / 2];
             {
-                Rphi t = {eps, 0.0};
+                double va = 0.0, vb = eps; // (v_j, v_(j+1))
                 _Pragma("unroll")
                 for (int j = 0; j < THETA_STRIDE; j += 2) {
-                    t = rphi_mul_theta_inv(F, t);
-                    const uint64_t w0 = low_word(t.v);
-                    t = rphi_mul_theta_inv(F, t);
-                    const uint64_t w1 = low_word(t.v);
+                    const uint64_t w0 = low_word(vb);
+                    va = F.red(fma(-3.0, vb, -va));
+                    const uint64_t w1 = low_word(va);
+                    vb = F.red(fma(-3.0, va, -vb));
                     uint64_t pair = w0 | (w1 << 32);
                     asm volatile("" : "+v"(pair)); // keep the words packed in one register pair
                     Tw[j / 2] = pair;
@@ -618,7 +619,9 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
                 ++idx;
                 record_hits(J, idx, p, x.u == eps && x.v == 0.0);
             }
-            // S(ne-1) for the test hook, by the doubling formulas
+            // S(ne-1), by the doubling formulas, only for the test hook that
+            // asks for final residues
+            if (!J.final_res) return 0;
             double fa, fb;
             fib_pair_fp(F, ne - 1, fa, fb); // (F(ne-1), F(ne))
             if constexpr (FAM == FAM_FIBONACCI) return F.canon(fa, p);
