@@ -354,7 +354,8 @@ def serve_block(args, eng, ctd, nat, jobs, cjobs, batched_digests):
     mx = jobs[0].max_factors
 
     def serve(sub, arrivals):
-        srv = ctd.Server(eng, fresh_buffers=True)
+        srv = ctd.Server(eng, fresh_buffers=True, streams=args.serve_streams, depth=args.serve_depth,
+                         tile_length=(1 << args.serve_tile_bits) if args.serve_tile_bits else 0)
         lat, dig, tickets = [0.0] * len(sub), [None] * len(sub), [None] * len(sub)
         ready = threading.Semaphore(0)
 
@@ -402,6 +403,8 @@ def serve_block(args, eng, ctd, nat, jobs, cjobs, batched_digests):
         "capacity_jobs_per_s": capacity,
         "capacity_updates_per_s": cap_st.updates / cap_s,
         "capacity_sample": f"the {len(cjobs)} mixed jobs submitted at once",
+        "server": {"streams": args.serve_streams, "depth": args.serve_depth or 2,
+                   "tile_length": 1 << (args.serve_tile_bits or 26)},
         "capacity_identical_to_batched": cap_dig == batched_digests,
         "load": args.serve_load,
         "offered_jobs_per_s": args.serve_load * capacity,
@@ -1003,9 +1006,11 @@ def main() -> int:
     ap.add_argument("--serve-jobs", type=int, default=10000, help="serve: jobs per offered load")
     ap.add_argument("--loads", default="0.5,0.7,0.9", help="serve: offered loads (fractions of capacity)")
     ap.add_argument("--capacity-jobs", type=int, default=2000, help="serve: jobs of the capacity measurement")
-    ap.add_argument("--serve-streams", type=int, default=3, help="serve: streams (resident prime slots)")
+    ap.add_argument("--serve-streams", type=int, default=3,
+                    help="serve and the serving block of mixed: streams (resident prime slots)")
     ap.add_argument("--serve-depth", type=int, default=0,
-                    help="serve: tiles of work each stream keeps in flight (0 = server default, 2)")
+                    help="serve and the serving block of mixed: tiles of work each stream keeps in flight "
+                         "(0 = server default, 2)")
     ap.add_argument("--serve-fifo", action="store_true",
                     help="serve: tiles go to the oldest job (default: alternately the oldest and the least remaining work)")
     ap.add_argument("--serve-cpu-jobs", type=int, default=12, help="serve: served jobs checked against the oracle")
@@ -1015,6 +1020,8 @@ def main() -> int:
     ap.add_argument("--serve-block-jobs", type=int, default=600,
                     help="mixed: jobs of the bounded serving stream (config 5 distribution; 0: skip)")
     ap.add_argument("--serve-load", type=float, default=0.9, help="mixed: offered load of the serving stream")
+    ap.add_argument("--serve-tile-bits", type=int, default=0,
+                    help="mixed: log2 of the serving tile length (0 = the server default, 2^26)")
     ap.add_argument("--tile-bits", type=int, default=0, help="mixed: log2 of the tile length (0 = engine default)")
     ap.add_argument("--batch-streams", type=int, default=0, help="mixed: streams of the batched form (0 = engine default)")
     ap.add_argument("--window-bits", type=int, default=30, help="mixed: log2 of each job's prime window")
