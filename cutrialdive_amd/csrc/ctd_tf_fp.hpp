@@ -44,7 +44,7 @@
 //   fibonacci / lucas         1.53 theta stride form, 5 pair form (6)
 //   tribonacci                4.5 (p < 2^44), 5.5 (11.25)
 //   smarandache r m + n + 1   7   (12.25), for p > 8 n1; 5 when p (0.75 m + 1) < 2^53,
-//                             6 when 0.75 p o < 2^53 (o the odd part of m)
+//                             6 when (p / 2 + n1) o < 2^53 (o the odd part of m)
 //
 // and the per-prime set-up (powers, doubling formulas, Smarandache jump) runs
 // on 6-instruction multiplies instead of ~40-instruction Montgomery products.
@@ -57,6 +57,8 @@ namespace ctd {
 constexpr uint64_t FP_PMAX = 1ull << 48;   // every prime of the wave must be below
 // the Smarandache short form needs p (0.75 m + 1) below 2^53 (margin for the rounded product)
 constexpr double SM_SHORT_LIMIT = 9007199254740992.0 - 1073741824.0;
+// above 1/2 + 2^-30: the reduced part of a Smarandache state is at most this times p
+constexpr double SM_EXACT_HALF = 0.5000001;
 
 struct Fmod {
     double p, invp;
@@ -734,12 +736,15 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
         // exact, and r' = (r m - k p) + c takes 4 instructions and the compare
         // (the same bound |r'| <= 0.75 p holds).
         //
-        // Exact-product form, for a segment where every lane has 0.75 p o <
-        // 2^53 with o the odd part of m (m = o 2^a): |r o| < 2^53, so r o is
-        // exact and so is r m = (r o) 2^a (a power-of-two scaling).  Then
-        // fma(-k, p, r m) has the exact value r m - k p, |.| <= (1/2 + 2^-31) p
-        // < 2^53, so it is exact, and r' = (r m - k p) + c takes 5 instructions
-        // and the compare.  Power-of-two bases (o = 1) always qualify.
+        // Exact-product form, for a segment where every lane has
+        // ((1/2 + 2^-30) p + n1) o < 2^53 with o the odd part of m (m = o 2^a).
+        // Every step of every form leaves |r| <= (1/2 + 2^-30) p + n1 (the
+        // reduced part, plus c < n1; the initial state is reduced), so
+        // |r o| < 2^53: r o is exact and so is r m = (r o) 2^a (a power-of-two
+        // scaling).  Then fma(-k, p, r m) has the exact value r m - k p,
+        // |.| <= (1/2 + 2^-30) p < 2^53, so it is exact, and
+        // r' = (r m - k p) + c takes 5 instructions and the compare.
+        // Power-of-two bases (o = 1) always qualify.
         const uint64_t bb = J.param;
         double r = sm_value_fp(F, bb, ns);
         record_hits(J, idx, p, r == 0.0);
@@ -757,6 +762,7 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
         }
         double mp = m * F.invp;
         const double pM = F.p * Fmod::MAGIC;
+        const double n1d = (double)J.n1; // c = n + 1 < n1 (n1 <= 2^32: JOB_SM_NARROW)
         uint64_t n = ns; // index of the current state
         while (n + 1 < ne) {
             // steps n -> n+1 .. seg_last -> seg_last+1 share m = b^digits(n+1)
@@ -772,8 +778,8 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
             const bool free_form = !(J.flags & JOB_SM_FULL_STEP);
             const bool short_form = free_form && !(J.flags & JOB_SM_NO_SHORT) &&
                                     __ballot(!(F.p * (0.75 * m + 1.0) < SM_SHORT_LIMIT)) == 0;
-            const bool exact_form =
-                !short_form && free_form && __ballot(!(F.p * (0.75 * om) < SM_SHORT_LIMIT)) == 0;
+            const bool exact_form = !short_form && free_form &&
+                                    __ballot(!((F.p * SM_EXACT_HALF + n1d) * om < SM_SHORT_LIMIT)) == 0;
             auto run = [&](auto step) {
                 uint32_t kk = 0;
                 for (; kk + UNROLL <= cnt; kk += UNROLL) {

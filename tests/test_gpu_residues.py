@@ -148,9 +148,10 @@ def _odd_part(m):
 
 @pytest.mark.parametrize("base", [10, 12, 6, 14, 2])
 def test_smarandache_exact_product_limit(engine, base):
-    # The exact-product Smarandache step needs 0.75 p o < 2^53 for the odd
-    # part o of the digit multiplier m (then r m is exact); a wave takes it
-    # when all its lanes qualify and the short step does not.  For each
+    # The exact-product Smarandache step needs (p / 2 + n1) o < 2^53 for the
+    # odd part o of the digit multiplier m (every state is at most
+    # (1/2 + 2^-30) p + n1 in size, so r m is exact); a wave takes it when all
+    # its lanes qualify and the short step does not.  For each
     # multiplier whose limit is below 2^48: a wave of primes just below the
     # limit, a wave just above it, and waves that interleave primes below it
     # with primes at 2-4 times it.  Power-of-two bases (o = 1) qualify at
@@ -163,7 +164,7 @@ def test_smarandache_exact_product_limit(engine, base):
     rng = random.Random(base + 100)
     sets = []
     for m in mults:
-        lim = int((2 ** 53 - 2 ** 30) / (0.75 * _odd_part(m)))
+        lim = int(((2 ** 53 - 2 ** 30) / _odd_part(m) - n1) / 0.5000001)
         if lim >= fp_max:
             continue
         below, above, far = [], [], []
