@@ -79,8 +79,8 @@ FAMILY_NAMES = {0: "mersenne", 1: "proth", 2: "riesel", 3: "smarandache", 4: "fi
                 6: "proth_fixed_n", 7: "riesel_fixed_n", 8: "tribonacci"}
 METRIC = "prime×index residue updates/s, batched vs plain on same GPU, bit-exact factors"
 HBM_PEAK_GBPS = 8000.0
-SM_SHORT_LIMIT = 2.0 ** 53 - 2.0 ** 30  # p (0.75 m + 1) below it: the short Smarandache step
-SM_EXACT_HALF = 0.5000001  # (SM_EXACT_HALF p + n1) o below SM_SHORT_LIMIT: the exact-product step
+SM_SHORT_LIMIT = 2.0 ** 53 - 2.0 ** 30  # the exactness limit of the Smarandache steps
+SM_EXACT_HALF = 0.5000001  # (SM_EXACT_HALF p + n1) o below SM_SHORT_LIMIT: the shifted Smarandache step
 
 
 def ops_table():
@@ -128,24 +128,23 @@ def odd_part(m: int) -> int:
 def window_ops(t, fam: str, lo: int, hi: int, n_items: int = 0, sm=None) -> float:
     """Prime-count-weighted loop ops per update over the window [lo, hi).
     sm = (base, n0, n1) for a Smarandache job: for each multiplier m its FP64
-    form takes the short step on the primes below 2^53 / (0.75 m + 1), else
-    the exact-product step on those with (p / 2 + n1) o < 2^53 (o the odd
-    part of m), else the general step."""
+    form takes a 5-instruction step on the primes with (p / 2 + n1) o < 2^53 (o
+    the odd part of m; every bench prime is above the base): the short step
+    where p (0.75 m + 1) < 2^53, else the shifted step; else the general
+    step."""
     cuts = {lo, hi} | {1 << b for b in (31, 35, 37, 41, 44, 45, 48) if lo < (1 << b) < hi}
     mults = sm_multipliers(*sm) if (fam == "smarandache" and sm) else []
-    lims = {m: int(SM_SHORT_LIMIT / (0.75 * m + 1)) for m, _ in mults}
     lims_x = {m: int((SM_SHORT_LIMIT / odd_part(m) - sm[2]) / SM_EXACT_HALF) for m, _ in mults} if mults else {}
-    cuts |= {x for x in list(lims.values()) + list(lims_x.values()) if lo < x < hi}
+    cuts |= {x for x in lims_x.values() if lo < x < hi}
     cuts = sorted(cuts)
     num = den = 0.0
     for a, b in zip(cuts, cuts[1:]):
         w = (b - a) / math.log(max(b, 16))
         ops = loop_ops(t, fam, b - 1, a, n_items)
         if mults and b - 1 < (1 << 48):
-            short = float(t["loop"][fam]["fp64_short"])
-            exact = float(t["loop"][fam]["fp64_exact"])
+            shift = float(t["loop"][fam]["fp64_shift"])
             steps = sum(c for _, c in mults)
-            ops = sum(c * (short if b <= lims[m] else exact if b <= lims_x[m] else ops) for m, c in mults) / steps
+            ops = sum(c * (shift if b <= lims_x[m] else ops) for m, c in mults) / steps
         num += w * ops
         den += w
     return num / den

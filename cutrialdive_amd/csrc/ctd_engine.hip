@@ -651,6 +651,7 @@ int run_job(ctd_hip_engine *E, const ctd_job *job, const ctd_run_opts *opts, con
                 if (bhi - 1 < FP_PMAX && !(opts && (opts->flags & CTD_RUN_NO_FP64))) J.flags |= JOB_FP64;
                 if (opts && (opts->flags & CTD_RUN_SM_FULL_STEP)) J.flags |= JOB_SM_FULL_STEP;
                 if (opts && (opts->flags & CTD_RUN_SM_NO_SHORT)) J.flags |= JOB_SM_NO_SHORT;
+                if (opts && (opts->flags & CTD_RUN_SM_NO_SHIFT)) J.flags |= JOB_SM_NO_SHIFT;
                 if (bhi - 1 < (1ull << 37)) J.flags |= JOB_P37;
                 J.flags |= stepped;
                 const uint32_t block = (opts && opts->block_size > 0) ? (uint32_t)opts->block_size : 256u;
@@ -1194,6 +1195,7 @@ int ctd_hip_residues(ctd_hip_engine *E, const ctd_job *job, const uint64_t *prim
         if (pmax < FP_PMAX && !(flags & CTD_RUN_NO_FP64)) J.flags |= JOB_FP64;
         if (flags & CTD_RUN_SM_FULL_STEP) J.flags |= JOB_SM_FULL_STEP;
         if (flags & CTD_RUN_SM_NO_SHORT) J.flags |= JOB_SM_NO_SHORT;
+        if (flags & CTD_RUN_SM_NO_SHIFT) J.flags |= JOB_SM_NO_SHIFT;
         if (pmax < (1ull << 37)) J.flags |= JOB_P37;
         if (flags & CTD_RUN_STEPPED) J.flags |= JOB_STEPPED;
         const uint64_t items = (nprimes + 63) & ~63ull;

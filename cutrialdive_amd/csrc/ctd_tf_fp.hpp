@@ -43,8 +43,9 @@
 //                             form of ctd_tf_fast.hpp solves once per prime
 //   fibonacci / lucas         1.53 theta stride form, 5 pair form (6)
 //   tribonacci                4.5 (p < 2^44), 5.5 (11.25)
-//   smarandache r m + n + 1   7   (12.25), for p > 8 n1; 5 when p (0.75 m + 1) < 2^53,
-//                             6 when (p / 2 + n1) o < 2^53 (o the odd part of m)
+//   smarandache r m + n + 1   7   (12.25), for p > 8 n1; 5 when p (0.75 m + 1) < 2^53
+//                             (short step) or (p / 2 + n1) o < 2^53 (shifted step,
+//                             o the odd part of m)
 //
 // and the per-prime set-up (powers, doubling formulas, Smarandache jump) runs
 // on 6-instruction multiplies instead of ~40-instruction Montgomery products.
@@ -453,6 +454,92 @@ __device__ __forceinline__ double fixedn_blocks(const JobArgs &J, uint64_t p, ui
     return x;
 }
 
+// A wave-uniform value moved to scalar registers (the compiler keeps values
+// derived from FP64 arithmetic in vector registers even when they are uniform).
+__device__ __forceinline__ uint64_t scalar_u64(uint64_t x)
+{
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// The largest i with i o < 2^(e+1), for an odd o < 2^21 and e <= 52, given
+// inv_o = 1/o rounded.  X = 2^(e+1) - 1 is exact in a double and X inv_o is
+// within 2^-52 X/o < 1 of X/o (o >= 3; exact for o = 1), so the truncated
+// quotient is off by at most one and one exact check corrects it.
+__device__ __forceinline__ uint64_t binade_last(uint32_t e, uint64_t o, double inv_o)
+{
+    const uint64_t X = (2ull << e) - 1;
+    uint64_t q = scalar_u64((uint64_t)((double)X * inv_o));
+    if (q * o > X) --q;
+    else if ((q + 1) * o <= X) ++q;
+    return q;
+}
+
+// Smarandache index steps.  step(v, s) advances the lane state v and the
+// wave-uniform scalar sequence s (a double's bit pattern, stepped on the
+// scalar unit) by one index; the step numbered kk (from 0) tests index
+// at0 + kk by v == 0 afterwards.  With PRODUCT the steps first run in blocks
+// of 16 with one wave-wide test per block: every state is a nonzero integer of
+// size below 2^48 unless it is a hit, so the product of a block's 16 states is
+// 0 exactly when one of them is (at most 2^768: no overflow; at least 1 in
+// size: no underflow).  The product costs the VALU what the compares cost,
+// but the scalar unit sees one mask per block instead of one per index.  A
+// block with a zero product is replayed from its first state with one compare
+// per index.  (Measured: the 4-instruction shifted step gains from it, the
+// longer steps do not, so they take blocks of 4 with per-index masks.)
+template <bool PRODUCT, typename Step>
+__device__ __forceinline__ void sm_blocks(const JobArgs &J, uint64_t p, uint64_t at0, uint32_t cnt, double &v,
+                                          uint64_t &s, Step step)
+{
+    uint32_t kk = 0;
+    for (; PRODUCT && kk + 16 <= cnt; kk += 16) {
+        const double v0 = v;
+        const uint64_t s0 = s;
+        step(v, s);
+        double prod = v;
+        _Pragma("unroll")
+        for (int j = 1; j < 16; ++j) {
+            step(v, s);
+            prod *= v;
+        }
+        if (__ballot(prod == 0.0)) {
+            const uint64_t at = hit_base(at0, kk);
+            double y = opaque_vgpr(v0);
+            uint64_t sy = s0;
+            for (int j = 0; j < 16; ++j) {
+                step(y, sy);
+                const uint64_t mk = __ballot(y == 0.0);
+                if (mk) record_hits_mask(J, at + (uint32_t)j, p, (mk >> __lane_id()) & 1, mk);
+            }
+        }
+    }
+    // the rest: blocks of 4 with one scalar branch, then single steps
+    for (; kk + 4 <= cnt; kk += 4) {
+        step(v, s);
+        const uint64_t m0 = __ballot(v == 0.0);
+        step(v, s);
+        const uint64_t m1 = __ballot(v == 0.0);
+        step(v, s);
+        const uint64_t m2 = __ballot(v == 0.0);
+        step(v, s);
+        const uint64_t m3 = __ballot(v == 0.0);
+        if (m0 | m1 | m2 | m3) {
+            const uint32_t lane = __lane_id();
+            const uint64_t at = hit_base(at0, kk);
+            if (m0) record_hits_mask(J, at, p, (m0 >> lane) & 1, m0);
+            if (m1) record_hits_mask(J, at + 1, p, (m1 >> lane) & 1, m1);
+            if (m2) record_hits_mask(J, at + 2, p, (m2 >> lane) & 1, m2);
+            if (m3) record_hits_mask(J, at + 3, p, (m3 >> lane) & 1, m3);
+        }
+    }
+    for (; kk < cnt; ++kk) {
+        step(v, s);
+        const uint64_t mk = __ballot(v == 0.0);
+        if (mk) record_hits_mask(J, hit_base(at0, kk), p, (mk >> __lane_id()) & 1, mk);
+    }
+}
+
 // The FP64 form of Fast<FAM>::run for a wave whose primes are all below
 // FP_PMAX (and, for smarandache, above 8 n1).  Returns the plain
 // residue S(ne-1) mod p for the test hook.
@@ -730,6 +817,11 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
         // 2^(e+1) still gives the exact value), so an integer add steps it and
         // the VALU does 7 instructions per index, the compare included.
         //
+        // The short form runs where its bound holds, the shifted form below
+        // elsewhere within the exact-product bound; the exact-product form
+        // runs in its place when it is switched off (JOB_SM_NO_SHIFT, a test
+        // hook) or where it does not apply (index 0, a prime dividing b).
+        //
         // Short form, for a segment where every lane has p (0.75 m + 1) < 2^53:
         // then |k p| < 2^53, so w = fma(k + M, -p, M p) = -k p is exact (M the
         // rounding constant, M p exact for p < 2^50), r m + w = r m - k p is
@@ -763,75 +855,114 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
         double mp = m * F.invp;
         const double pM = F.p * Fmod::MAGIC;
         const double n1d = (double)J.n1; // c = n + 1 < n1 (n1 <= 2^32: JOB_SM_NARROW)
+        double inv_om = 1.0 / om;
         uint64_t n = ns; // index of the current state
+        // The shifted form holds t = r - n instead of r (`shifted`); index n
+        // itself is then not tested yet: its test is the next step's, or the
+        // one that follows restoring r.
+        bool shifted = false;
+        double t = 0.0;
         while (n + 1 < ne) {
             // steps n -> n+1 .. seg_last -> seg_last+1 share m = b^digits(n+1)
-            // (digits grow when n+1 reaches nxt) and the binade of c = n+1
-            const uint64_t c0 = n + 1;
-            const uint32_t e = 63 - __clzll(c0);
+            // (digits grow when n+1 reaches nxt); each form also keeps its
+            // scalar sequence inside one binade
             uint64_t seg_last = nxt - 2 < ne - 2 ? nxt - 2 : ne - 2;
-            if ((2ull << e) - 2 < seg_last) seg_last = (2ull << e) - 2;
-            const uint32_t cnt = (uint32_t)(seg_last - n + 1); // below 2^32: n1 <= 2^32
-            const uint64_t delta = 1ull << (52 - e);
-            uint64_t cb = __double_as_longlong((double)c0);
             // one wave-uniform choice per segment (m is fixed inside it)
             const bool free_form = !(J.flags & JOB_SM_FULL_STEP);
+            const bool exact_ok = free_form && __ballot(!((F.p * SM_EXACT_HALF + n1d) * om < SM_SHORT_LIMIT)) == 0;
+            // the short step, where it is exact, runs as fast as the shifted
+            // one and needs no state change (measured: 1-2 % faster)
             const bool short_form = free_form && !(J.flags & JOB_SM_NO_SHORT) &&
                                     __ballot(!(F.p * (0.75 * m + 1.0) < SM_SHORT_LIMIT)) == 0;
-            const bool exact_form = !short_form && free_form &&
-                                    __ballot(!((F.p * SM_EXACT_HALF + n1d) * om < SM_SHORT_LIMIT)) == 0;
-            auto run = [&](auto step) {
-                uint32_t kk = 0;
-                for (; kk + UNROLL <= cnt; kk += UNROLL) {
-                    step(); const uint64_t m0 = __ballot(r == 0.0);
-                    step(); const uint64_t m1 = __ballot(r == 0.0);
-                    step(); const uint64_t m2 = __ballot(r == 0.0);
-                    step(); const uint64_t m3 = __ballot(r == 0.0);
-                    if (m0 | m1 | m2 | m3) {
-                        const uint32_t lane = __lane_id();
-                        const uint64_t at = hit_base(idx, kk);
-                        if (m0) record_hits_mask(J, at + 1, p, (m0 >> lane) & 1, m0);
-                        if (m1) record_hits_mask(J, at + 2, p, (m1 >> lane) & 1, m1);
-                        if (m2) record_hits_mask(J, at + 3, p, (m2 >> lane) & 1, m2);
-                        if (m3) record_hits_mask(J, at + 4, p, (m3 >> lane) & 1, m3);
-                    }
+            if (!short_form && exact_ok && n != 0 && !(J.flags & JOB_SM_NO_SHIFT) && __ballot(p <= bb) == 0) {
+                // Shifted form: the state is t = r - n (n the last appended
+                // number), so one fma forms r m = (t + n) m = t m + (n m) with the
+                // wave-uniform n m from the scalar unit, exactly (the exact-product
+                // bound: |r| o < 2^53 and n o < 2^53).  t' = r m - k p is exact
+                // (|.| <= (1/2 + 2^-30) p) and equals S(n+1) - (n+1) mod p.  The
+                // test of index n needs no appended number: S(n) = 0 mod p <=>
+                // S(n) m = 0 <=> t' == 0 (m = b^d is prime to p > b).  So a step
+                // is 4 instructions and the test, and the tests lag one index:
+                // the first step after entering the form repeats the test of
+                // index n (skipped), and the last index is tested when r = t + n
+                // is restored.  n m is stepped as a bit pattern inside the
+                // binade of n o (m = o 2^a), so a segment also ends where n o
+                // leaves its binade; t carries over to the next segment.
+                const uint64_t on = scalar_u64((uint64_t)om);
+                const uint32_t e2 = 63 - __clzll(n * on);
+                const uint64_t last_i = binade_last(e2, on, inv_om);
+                if (last_i < seg_last) seg_last = last_i;
+                const uint32_t cs = (uint32_t)(seg_last - n + 1);
+                const double mv = opaque_vgpr(m);
+                // n < n1 <= 2^32; the pattern is made scalar (m lives in a vector register)
+                uint64_t cmb = scalar_u64(__double_as_longlong((double)(uint32_t)n * m));
+                const uint64_t dcm = on << (52 - e2);
+                auto step = [&](double &v, uint64_t &s) {
+                    const double rm = fma(v, mv, __longlong_as_double(s));
+                    const double k = fma(rm, F.invp, Fmod::MAGIC) - Fmod::MAGIC;
+                    v = fma(-k, F.p, rm);
+                    s += dcm;
+                };
+                // entering the form: t = r - n, and the first step repeats the
+                // test of index n (recorded already), so it runs untested
+                const uint32_t done = shifted ? 0u : 1u;
+                if (done) {
+                    t = r - (double)(uint32_t)n;
+                    step(t, cmb);
+                    shifted = true;
                 }
-                idx += kk;
-                for (; kk < cnt; ++kk) {
-                    step();
-                    ++idx;
-                    const uint64_t mk = __ballot(r == 0.0);
-                    if (mk) record_hits_mask(J, idx, p, (mk >> __lane_id()) & 1, mk);
-                }
-            };
-            if (short_form) {
-                run([&](void) {
-                    const double k1 = fma(r, mp, Fmod::MAGIC);
-                    r = fma(r, m, fma(k1, -F.p, pM)) + __longlong_as_double(cb);
-                    cb += delta;
-                });
-            } else if (exact_form) {
-                run([&](void) {
-                    const double rm = r * m;
-                    const double k = fma(r, mp, Fmod::MAGIC) - Fmod::MAGIC;
-                    r = fma(-k, F.p, rm) + __longlong_as_double(cb);
-                    cb += delta;
-                });
+                // (wave-uniform values; the compiler cannot prove it for `shifted`)
+                sm_blocks<true>(J, p, scalar_u64(idx + done), __builtin_amdgcn_readfirstlane(cs - done), t, cmb, step);
+                n = seg_last + 1;
+                idx += cs;
             } else {
-                run([&](void) {
-                    const double k = fma(r, mp, Fmod::MAGIC) - Fmod::MAGIC;
-                    const double t = fma(r, m, -(k * ph));
-                    r = fma(-k, pl, t + __longlong_as_double(cb));
-                    cb += delta;
-                });
+                if (shifted) { // back to r; index n is tested here
+                    r = t + (double)(uint32_t)n;
+                    record_hits(J, idx, p, r == 0.0);
+                    shifted = false;
+                }
+                // the appended number c = n+1 is stepped inside its binade
+                const uint64_t c0 = n + 1;
+                const uint32_t e = 63 - __clzll(c0);
+                if ((2ull << e) - 2 < seg_last) seg_last = (2ull << e) - 2;
+                const uint32_t cnt = (uint32_t)(seg_last - n + 1); // below 2^32: n1 <= 2^32
+                const uint64_t delta = 1ull << (52 - e);
+                uint64_t cb = __double_as_longlong((double)c0);
+                if (short_form) {
+                    sm_blocks<false>(J, p, idx + 1, cnt, r, cb, [&](double &v, uint64_t &s) {
+                        const double k1 = fma(v, mp, Fmod::MAGIC);
+                        v = fma(v, m, fma(k1, -F.p, pM)) + __longlong_as_double(s);
+                        s += delta;
+                    });
+                } else if (exact_ok) {
+                    sm_blocks<false>(J, p, idx + 1, cnt, r, cb, [&](double &v, uint64_t &s) {
+                        const double rm = v * m;
+                        const double k = fma(v, mp, Fmod::MAGIC) - Fmod::MAGIC;
+                        v = fma(-k, F.p, rm) + __longlong_as_double(s);
+                        s += delta;
+                    });
+                } else {
+                    sm_blocks<false>(J, p, idx + 1, cnt, r, cb, [&](double &v, uint64_t &s) {
+                        const double k = fma(v, mp, Fmod::MAGIC) - Fmod::MAGIC;
+                        const double t = fma(v, m, -(k * ph));
+                        v = fma(-k, pl, t + __longlong_as_double(s));
+                        s += delta;
+                    });
+                }
+                idx += cnt;
+                n = seg_last + 1;
             }
-            n = seg_last + 1;
             if (n + 1 == nxt) { // the next appended number has one more digit
                 m *= (double)bb;
                 om *= ob;
+                inv_om = 1.0 / om;
                 mp = m * F.invp;
                 nxt = sat_mul(nxt, bb);
             }
+        }
+        if (shifted) {
+            r = t + (double)(uint32_t)n;
+            record_hits(J, idx, p, r == 0.0);
         }
         return F.canon(r, p);
     } else {

@@ -86,8 +86,11 @@ typedef struct ctd_run_opts {
 #define CTD_RUN_SM_FULL_STEP 4u /* Smarandache FP64 form: the general 7-instruction step only (no short step) */
 #define CTD_RUN_STEPPED 8u      /* fixed-n families: step every index (riesel_proth_fixed_n.hpp:142-151)
                                    instead of solving k = -+2^-n (mod p) once per prime */
-#define CTD_RUN_SM_NO_SHORT 16u /* Smarandache FP64 form: no short step (the 6-instruction exact-product
-                                   step where r b^digits is exact, else the general step); test hook */
+#define CTD_RUN_SM_NO_SHORT 16u /* Smarandache FP64 form: no short step (where r b^digits is exact the
+                                   shifted step, or with CTD_RUN_SM_NO_SHIFT the 6-instruction
+                                   exact-product step; else the general step); test hook */
+#define CTD_RUN_SM_NO_SHIFT 32u /* Smarandache FP64 form: no shifted step (the short, exact-product or
+                                   general step, as CTD_RUN_SM_NO_SHORT allows); test hook */
 
 typedef struct ctd_hip_engine ctd_hip_engine;
 

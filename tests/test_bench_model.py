@@ -1,8 +1,8 @@
 """The VALU-per-update model that bench.py's roofline uses (CPU only).
 
 The model must pick, for each prime window, the loop form the kernels pick:
-the Smarandache short step where p (0.75 m + 1) < 2^53 for the digit
-multiplier m, the shift stride below 2^37 and the word stride above for the
+the Smarandache short or shifted step (5 instructions) where (p / 2 + n1) o
+< 2^53 for the odd part o of the digit multiplier m, the shift stride below 2^37 and the word stride above for the
 doubling families, the Tribonacci lazy form below 2^44."""
 import os
 import sys
@@ -31,18 +31,21 @@ def test_sm_multipliers_match_brute_force():
         assert got == want, (b, n0, n1)
 
 
-def test_smarandache_short_and_long_windows():
+def test_smarandache_shifted_and_general_windows():
     t = bench.ops_table()
-    short, full = t["loop"]["smarandache"]["fp64_short"], t["loop"]["smarandache"]["fp64"]
-    # cfg2: m <= 1000, p < 2^36: every step short
-    assert bench.window_ops(t, "smarandache", 1 << 32, 1 << 36, 990, (10, 10, 1000)) == pytest.approx(short)
-    # m = 10^5 at p near 2^43: p (0.75 m + 1) > 2^53, every step long
+    shift, full = t["loop"]["smarandache"]["fp64_shift"], t["loop"]["smarandache"]["fp64"]
+    # cfg2: m <= 1000, p < 2^36: every step shifted
+    assert bench.window_ops(t, "smarandache", 1 << 32, 1 << 36, 990, (10, 10, 1000)) == pytest.approx(shift)
+    # base 10 at p near 2^43: m = 10^5 has odd part 5^5, (p / 2 + n1) 5^5 < 2^53: shifted
+    assert bench.window_ops(t, "smarandache", 1 << 42, (1 << 42) + (1 << 30), 1000,
+                            (10, 20000, 21000)) == pytest.approx(shift)
+    # base 13, m = 13^4 = 28561 at p near 2^43: (p / 2 + n1) m > 2^53, every step general
     assert bench.window_ops(t, "smarandache", 1 << 43, (1 << 43) + (1 << 30), 1000,
-                            (10, 20000, 21000)) == pytest.approx(full)
+                            (13, 3000, 4000)) == pytest.approx(full)
     # a window straddling the limit of its single multiplier lies strictly between
-    lim = int(bench.SM_SHORT_LIMIT / (0.75 * 10 ** 4 + 1))
-    mid = bench.window_ops(t, "smarandache", lim - (1 << 30), lim + (1 << 30), 1000, (10, 2000, 3000))
-    assert short < mid < full
+    lim = int((bench.SM_SHORT_LIMIT / 13 ** 4 - 4000) / bench.SM_EXACT_HALF)
+    mid = bench.window_ops(t, "smarandache", lim - (1 << 30), lim + (1 << 30), 1000, (13, 3000, 4000))
+    assert shift < mid < full
 
 
 def test_stride_forms_by_prime_size():

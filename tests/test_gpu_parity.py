@@ -397,20 +397,60 @@ def test_theta_form_windows(engine, seq, n0, n1, f0, f1, chunk):
     assert_same(g, gc, seq, 0, n0, n1, f0, f1)
 
 
-@pytest.mark.parametrize("base", [10, 2, 16])
+# default (the short step where it is exact), the shifted step, the
+# exact-product step, the general step
+SM_FORM_FLAGS = (0, nat.CTD_RUN_SM_NO_SHORT, nat.CTD_RUN_SM_NO_SHIFT | nat.CTD_RUN_SM_NO_SHORT,
+                 nat.CTD_RUN_SM_FULL_STEP)
+
+
+@pytest.mark.parametrize("base", [10, 2, 16, 3])
 def test_smarandache_step_forms_with_hits(engine, base):
     # primes in [2^20, 2^21) are above 8 n1, so the FP64 form runs; p m is far
-    # below 2^53, so it takes the short step by default,
-    # CTD_RUN_SM_NO_SHORT runs the exact-product step (p o is far below 2^53
-    # too) and CTD_RUN_SM_FULL_STEP the general step on the same primes.
-    # About 250 hits per base.
+    # below 2^53, so it takes the short step by default; CTD_RUN_SM_NO_SHORT
+    # runs the shifted step ((p / 2 + n1) o is far below 2^53 too), adding
+    # CTD_RUN_SM_NO_SHIFT the exact-product step, and CTD_RUN_SM_FULL_STEP
+    # the general step on the same primes.  About 250 hits per base.
     n0, n1 = 1, 5000
     f0, f1 = 1 << 20, 1 << 21
     r = o.trial_factor(o.SMARANDACHE, base, n0, n1, f0, f1, threads=16)
     assert sum(len(x) for x in r) > 100
-    for flags in (0, nat.CTD_RUN_SM_NO_SHORT, nat.CTD_RUN_SM_FULL_STEP):
+    for flags in SM_FORM_FLAGS:
         g, _, _ = gpu_lists(engine, o.SMARANDACHE, base, n0, n1, f0, f1, flags=flags)
         assert g == r, flags
+
+
+@pytest.mark.parametrize("base", [10, 3, 7, 2, 16, 12])
+def test_smarandache_shifted_step_segment_edges(engine, base):
+    """The shifted step tests index n after the step n -> n+1, one index
+    late: the first step of a segment repeats a tested index (skipped) and the
+    last index of the segment is tested after the state is restored.  Each of
+    the first hits of the window is placed at the start, at the second index,
+    at the last index and at every position of a 16-index block of a job, and
+    the jobs must equal the oracle in every step form."""
+    f0, f1 = 1 << 14, 1 << 16  # above 8 n1 for n1 <= 2048: the FP64 forms run
+    r = o.trial_factor(o.SMARANDACHE, base, 1, 2048, f0, f1)
+    hits = [1 + i for i, x in enumerate(r) if x][:12]
+    assert len(hits) >= 8
+    ranges = set()
+    for h in hits:
+        ranges |= {(h, h + 40), (max(1, h - 1), h + 40), (max(1, h - 2), h + 1), (max(1, h - 17), h + 1)}
+        ranges |= {(max(1, h - 1 - j), h + 19) for j in (0, 1, 7, 14, 15, 16)}
+    for n0, n1 in sorted(ranges):
+        want = o.trial_factor(o.SMARANDACHE, base, n0, n1, f0, f1)
+        for flags in SM_FORM_FLAGS:
+            g, _, _ = gpu_lists(engine, o.SMARANDACHE, base, n0, n1, f0, f1, flags=flags)
+            assert g == want, (n0, n1, flags)
+
+
+def test_smarandache_prime_dividing_the_base(engine):
+    # p = 61 divides every digit multiplier of base 61: the shifted step's test
+    # (S(n) m = 0 <=> S(n) = 0) needs p > b, so waves holding such a prime
+    # take the other steps.  p > 8 n1 keeps the FP64 form.
+    for n0, n1 in ((1, 5), (2, 7), (3, 4)):
+        want = o.trial_factor(o.SMARANDACHE, 61, n0, n1, 41, 4000)
+        for flags in SM_FORM_FLAGS:
+            g, _, _ = gpu_lists(engine, o.SMARANDACHE, 61, n0, n1, 41, 4000, flags=flags)
+            assert g == want, (n0, n1, flags)
 
 
 def _canon(v, p):

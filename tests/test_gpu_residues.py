@@ -107,7 +107,10 @@ def test_smarandache_short_step_limit(engine, base):
     # qualify.  For each multiplier of the run: a wave of primes just below
     # the limit, a wave just above it, and waves that interleave primes below
     # the limit with primes at 2-4 times it (where the short step would be
-    # inexact).  Final residues are compared with and without the short step.
+    # inexact).  Final residues are compared by default (the short step below
+    # its limit, the shifted step above it), with the shifted step wherever it
+    # is exact (CTD_RUN_SM_NO_SHORT), without it (CTD_RUN_SM_NO_SHIFT: the
+    # exact-product step above the limit) and with the general step only.
     n0, n1 = 5, 2100
     mults = sorted({_digits_multiplier(base, n) for n in range(n0, n1)})
     fp_max = (1 << 48) - 1
@@ -134,7 +137,7 @@ def test_smarandache_short_step_limit(engine, base):
         mixed = [x for pair in zip(below[64:], far) for x in pair] if far else below[64:]
         primes = below[:64] + above + mixed
         want = o.residues_last(o.SMARANDACHE, base, n0, n1 - n0, primes)
-        for flags in (0, nat.CTD_RUN_SM_FULL_STEP):
+        for flags in (0, nat.CTD_RUN_SM_NO_SHORT, nat.CTD_RUN_SM_NO_SHIFT, nat.CTD_RUN_SM_FULL_STEP):
             got = engine.residues(o.SMARANDACHE, base, n0, n1, primes, flags)
             bad = np.nonzero(got != want)[0]
             assert len(bad) == 0, (m, flags, [(primes[i], int(got[i]), int(want[i])) for i in bad[:5]])
@@ -146,7 +149,7 @@ def _odd_part(m):
     return m
 
 
-@pytest.mark.parametrize("base", [10, 12, 6, 14, 2])
+@pytest.mark.parametrize("base", [10, 12, 6, 14, 2, 3])
 def test_smarandache_exact_product_limit(engine, base):
     # The exact-product Smarandache step needs (p / 2 + n1) o < 2^53 for the
     # odd part o of the digit multiplier m (every state is at most
@@ -155,9 +158,12 @@ def test_smarandache_exact_product_limit(engine, base):
     # multiplier whose limit is below 2^48: a wave of primes just below the
     # limit, a wave just above it, and waves that interleave primes below it
     # with primes at 2-4 times it.  Power-of-two bases (o = 1) qualify at
-    # every prime below 2^48: waves spread over [2^40, 2^48).  Final residues
-    # with every step allowed, without the short step (CTD_RUN_SM_NO_SHORT)
-    # and with the general step only.
+    # every prime below 2^48: waves spread over [2^40, 2^48).  The shifted
+    # step has the same limit.  Final residues with every step allowed, with
+    # the shifted step wherever it is exact (CTD_RUN_SM_NO_SHORT), without it
+    # (CTD_RUN_SM_NO_SHIFT: the short step below its own limit, else the
+    # exact-product step), without the shifted and the short step, and with
+    # the general step only.
     n0, n1 = 5, 2100
     mults = sorted({_digits_multiplier(base, n) for n in range(n0, n1)})
     fp_max = (1 << 48) - 1
@@ -190,7 +196,8 @@ def test_smarandache_exact_product_limit(engine, base):
     assert sets
     for m, primes in sets:
         want = o.residues_last(o.SMARANDACHE, base, n0, n1 - n0, primes)
-        for flags in (0, nat.CTD_RUN_SM_NO_SHORT, nat.CTD_RUN_SM_FULL_STEP):
+        for flags in (0, nat.CTD_RUN_SM_NO_SHORT, nat.CTD_RUN_SM_NO_SHIFT,
+                      nat.CTD_RUN_SM_NO_SHIFT | nat.CTD_RUN_SM_NO_SHORT, nat.CTD_RUN_SM_FULL_STEP):
             got = engine.residues(o.SMARANDACHE, base, n0, n1, primes, flags)
             bad = np.nonzero(got != want)[0]
             assert len(bad) == 0, (m, flags, [(primes[i], int(got[i]), int(want[i])) for i in bad[:5]])
