@@ -4,6 +4,8 @@ issue and autotuned launch shape; the same graph-built batch run several
 times in a row gives the same buffers each time."""
 from __future__ import annotations
 
+import os
+
 import numpy as np
 import pytest
 
@@ -86,6 +88,29 @@ def test_mixed_jobs_first_window_slice(engine):
     for k, (s, (c, f)) in enumerate(zip(jobs, res)):
         rc, rf = oracle_of(s)
         assert_same(c, f, rc, rf, s.max_factors, s.max_factors, (k, s))
+
+
+@pytest.mark.parametrize("where", ["own", "low"])
+def test_mixed_jobs_full_index_ranges(engine, where):
+    """All 256 jobs of the mixed batch with their full index ranges (10^2 to
+    10^5 indices, starts up to 10^5), each on 2^15 integers: the first ones of
+    its own prime window in [2^38, 2^43) ("own"), or [2^24, 2^24 + 2^15)
+    ("low", where the FP64 forms run too and 128 of the jobs have hits, 639 in
+    all).  The factors buffers equal the oracle's."""
+    lo = 1 << 24
+    jobs = [s.with_window(s.f0, s.f0 + (1 << 15)) if where == "own" else s.with_window(lo, lo + (1 << 15))
+            for s in workloads.mixed_jobs()]
+    print("mixed jobs, full index ranges,", where, "windows", flush=True)
+    res, st = engine.run_batch([job_of(s) for s in jobs])
+    assert st.tiles >= len(jobs)
+    threads = min(16, os.cpu_count() or 1)
+    hits = 0
+    for k, (s, (c, f)) in enumerate(zip(jobs, res)):
+        rc, rf = oracle.trial_factor_raw(s.seq_id, s.param, s.n0, s.n1, s.f0, s.f1, s.max_factors, threads)
+        assert_same(c, f, rc, rf, s.max_factors, s.max_factors, (k, s))
+        hits += int(rc.sum())
+    if where == "low":
+        assert hits > 256
 
 
 def test_graph_replays_agree(engine):
