@@ -855,7 +855,7 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
         double mp = m * F.invp;
         const double pM = F.p * Fmod::MAGIC;
         const double n1d = (double)J.n1; // c = n + 1 < n1 (n1 <= 2^32: JOB_SM_NARROW)
-        double inv_om = 1.0 / om;
+        double inv_om = 0.0, inv_om_of = 0.0; // 1/om, and the om it belongs to
         uint64_t n = ns; // index of the current state
         // The shifted form holds t = r - n instead of r (`shifted`); index n
         // itself is then not tested yet: its test is the next step's, or the
@@ -869,12 +869,13 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
             uint64_t seg_last = nxt - 2 < ne - 2 ? nxt - 2 : ne - 2;
             // one wave-uniform choice per segment (m is fixed inside it)
             const bool free_form = !(J.flags & JOB_SM_FULL_STEP);
-            const bool exact_ok = free_form && __ballot(!((F.p * SM_EXACT_HALF + n1d) * om < SM_SHORT_LIMIT)) == 0;
             // the short step, where it is exact, runs as fast as the shifted
             // one and needs no state change (measured: 1-2 % faster)
             const bool short_form = free_form && !(J.flags & JOB_SM_NO_SHORT) &&
                                     __ballot(!(F.p * (0.75 * m + 1.0) < SM_SHORT_LIMIT)) == 0;
-            if (!short_form && exact_ok && n != 0 && !(J.flags & JOB_SM_NO_SHIFT) && __ballot(p <= bb) == 0) {
+            const bool exact_ok = !short_form && free_form &&
+                                  __ballot(!((F.p * SM_EXACT_HALF + n1d) * om < SM_SHORT_LIMIT)) == 0;
+            if (exact_ok && n != 0 && !(J.flags & JOB_SM_NO_SHIFT) && __ballot(p <= bb) == 0) {
                 // Shifted form: the state is t = r - n (n the last appended
                 // number), so one fma forms r m = (t + n) m = t m + (n m) with the
                 // wave-uniform n m from the scalar unit, exactly (the exact-product
@@ -888,6 +889,10 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
                 // is restored.  n m is stepped as a bit pattern inside the
                 // binade of n o (m = o 2^a), so a segment also ends where n o
                 // leaves its binade; t carries over to the next segment.
+                if (inv_om_of != om) { // 1/o, once per digit count
+                    inv_om = 1.0 / om;
+                    inv_om_of = om;
+                }
                 const uint64_t on = scalar_u64((uint64_t)om);
                 const uint32_t e2 = 63 - __clzll(n * on);
                 const uint64_t last_i = binade_last(e2, on, inv_om);
@@ -955,7 +960,6 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
             if (n + 1 == nxt) { // the next appended number has one more digit
                 m *= (double)bb;
                 om *= ob;
-                inv_om = 1.0 / om;
                 mp = m * F.invp;
                 nxt = sat_mul(nxt, bb);
             }
