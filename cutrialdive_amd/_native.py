@@ -28,6 +28,7 @@ CTD_RUN_NO_FP64 = 2
 CTD_RUN_SM_FULL_STEP = 4
 CTD_RUN_SM_NO_SHORT = 16
 CTD_RUN_SM_NO_SHIFT = 32
+CTD_RUN_SM_NO_PAIR = 64
 CTD_RUN_STEPPED = 8
 CTD_BATCH_HOST_BUFFERS = 1
 CTD_BATCH_NO_GRAPH = 2

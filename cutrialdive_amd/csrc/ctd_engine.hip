@@ -96,6 +96,45 @@ __global__ void __launch_bounds__(256) tf_fast_kernel(JobArgs J, const uint64_t 
     }
 }
 
+// Smarandache with two primes per lane: the lanes of a wave take the primes
+// base + l and base + 64 + l of a 128-prime group and run the FP64 steps of
+// both in one loop (run_fp_sm<2>), so the scalar stepping of the appended
+// numbers, the loop control and the segment choices serve two updates per
+// step.  A group that is not full, or whose primes do not all take the FP64
+// form, runs its primes one after the other.
+__global__ void __launch_bounds__(256) tf_sm_pair_kernel(JobArgs J, const uint64_t *__restrict__ primes,
+                                                          uint64_t nprimes, const unsigned long long *np_dev)
+{
+    nprimes = prime_count(nprimes, np_dev);
+    const uint64_t half = ((nprimes + 127) & ~127ull) / 2; // lanes of 128-prime groups per chunk
+    const uint64_t work = half * J.nchunks;
+    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < work;
+         w += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t c = __builtin_amdgcn_readfirstlane((uint32_t)(w / half));
+        const uint64_t j = w - c * half;
+        const uint64_t i0 = ((j & ~63ull) << 1) + (j & 63), i1 = i0 + 64;
+        if (i0 >= nprimes) continue;
+        const uint64_t ns = J.n0 + c * J.chunk;
+        uint64_t ne = ns + J.chunk;
+        if (ne > J.n1 || ne < ns) ne = J.n1;
+        const bool has1 = i1 < nprimes;
+        const uint64_t p0 = primes[i0], p1 = has1 ? primes[i1] : p0;
+        if (__ballot(!has1) == 0 && use_fp<FAM_SMARANDACHE>(J, p0, ne - ns - 1) &&
+            use_fp<FAM_SMARANDACHE>(J, p1, ne - ns - 1)) {
+            const uint64_t pp[2] = {p0, p1};
+            uint64_t fin[2];
+            run_fp_sm<2>(J, pp, ns, ne, ns - J.n0, fin);
+            if (J.final_res) {
+                J.final_res[i0] = fin[0];
+                J.final_res[i1] = fin[1];
+            }
+        } else {
+            Fast<FAM_SMARANDACHE>::run(J, p0, ns, ne, i0);
+            if (has1) Fast<FAM_SMARANDACHE>::run(J, p1, ns, ne, i1);
+        }
+    }
+}
+
 // Prime 2: S(n) mod 2 per index.
 __global__ void mod2_kernel(JobArgs J)
 {
@@ -481,6 +520,8 @@ void launch_tf(const JobArgs &J, const uint64_t *primes, uint64_t nprimes, const
     constexpr bool doubling = FAM == FAM_MERSENNE || FAM == FAM_PROTH || FAM == FAM_RIESEL;
     if (fast && doubling && (J.flags & JOB_P37))
         hipLaunchKernelGGL((tf_fast_kernel<FAM, doubling>), grid, block, 0, st, J, primes, nprimes, np_dev);
+    else if (fast && FAM == FAM_SMARANDACHE && !(J.flags & JOB_SM_NO_PAIR))
+        hipLaunchKernelGGL(tf_sm_pair_kernel, grid, block, 0, st, J, primes, nprimes, np_dev);
     else if (fast)
         hipLaunchKernelGGL((tf_fast_kernel<FAM, false>), grid, block, 0, st, J, primes, nprimes, np_dev);
     else
@@ -652,6 +693,7 @@ int run_job(ctd_hip_engine *E, const ctd_job *job, const ctd_run_opts *opts, con
                 if (opts && (opts->flags & CTD_RUN_SM_FULL_STEP)) J.flags |= JOB_SM_FULL_STEP;
                 if (opts && (opts->flags & CTD_RUN_SM_NO_SHORT)) J.flags |= JOB_SM_NO_SHORT;
                 if (opts && (opts->flags & CTD_RUN_SM_NO_SHIFT)) J.flags |= JOB_SM_NO_SHIFT;
+                if (opts && (opts->flags & CTD_RUN_SM_NO_PAIR)) J.flags |= JOB_SM_NO_PAIR;
                 if (bhi - 1 < (1ull << 37)) J.flags |= JOB_P37;
                 J.flags |= stepped;
                 const uint32_t block = (opts && opts->block_size > 0) ? (uint32_t)opts->block_size : 256u;
@@ -1196,6 +1238,7 @@ int ctd_hip_residues(ctd_hip_engine *E, const ctd_job *job, const uint64_t *prim
         if (flags & CTD_RUN_SM_FULL_STEP) J.flags |= JOB_SM_FULL_STEP;
         if (flags & CTD_RUN_SM_NO_SHORT) J.flags |= JOB_SM_NO_SHORT;
         if (flags & CTD_RUN_SM_NO_SHIFT) J.flags |= JOB_SM_NO_SHIFT;
+        if (flags & CTD_RUN_SM_NO_PAIR) J.flags |= JOB_SM_NO_PAIR;
         if (pmax < (1ull << 37)) J.flags |= JOB_P37;
         if (flags & CTD_RUN_STEPPED) J.flags |= JOB_STEPPED;
         const uint64_t items = (nprimes + 63) & ~63ull;

@@ -46,6 +46,7 @@ constexpr uint32_t JOB_P37 = 8;       // every prime below 2^37: the doubling sh
 constexpr uint32_t JOB_STEPPED = 16;  // fixed-n families: step every index (no closed form, ctd_tf_fast.hpp)
 constexpr uint32_t JOB_SM_NO_SHORT = 32; // smarandache FP64 form: skip the short step (exact-product or general)
 constexpr uint32_t JOB_SM_NO_SHIFT = 64; // smarandache FP64 form: skip the shifted step (short, exact-product or general)
+constexpr uint32_t JOB_SM_NO_PAIR = 128; // smarandache: one prime per lane (no tf_sm_pair_kernel)
 
 // Job constants visible to every lane.
 struct JobArgs {

@@ -487,56 +487,302 @@ __device__ __forceinline__ uint64_t binade_last(uint32_t e, uint64_t o, double i
 // but the scalar unit sees one mask per block instead of one per index.  A
 // block with a zero product is replayed from its first state with one compare
 // per index.  (Measured: the 4-instruction shifted step gains from it, the
-// longer steps do not, so they take blocks of 4 with per-index masks.)
-template <bool PRODUCT, typename Step>
-__device__ __forceinline__ void sm_blocks(const JobArgs &J, uint64_t p, uint64_t at0, uint32_t cnt, double &v,
-                                          uint64_t &s, Step step)
+// longer steps do not, so they take blocks of 4 with per-index masks.)  The
+// K primes of a lane share s and the loop control.
+template <int K, bool PRODUCT, typename Step>
+__device__ __forceinline__ void sm_blocks(const JobArgs &J, const uint64_t (&p)[K], uint64_t at0, uint32_t cnt,
+                                          double (&v)[K], uint64_t &s, Step step)
 {
     uint32_t kk = 0;
     for (; PRODUCT && kk + 16 <= cnt; kk += 16) {
-        const double v0 = v;
+        double v0[K], prod[K];
+        _Pragma("unroll")
+        for (int q = 0; q < K; ++q) v0[q] = v[q];
         const uint64_t s0 = s;
         step(v, s);
-        double prod = v;
+        _Pragma("unroll")
+        for (int q = 0; q < K; ++q) prod[q] = v[q];
         _Pragma("unroll")
         for (int j = 1; j < 16; ++j) {
             step(v, s);
-            prod *= v;
+            _Pragma("unroll")
+            for (int q = 0; q < K; ++q) prod[q] *= v[q];
         }
-        if (__ballot(prod == 0.0)) {
+        bool zero = false;
+        _Pragma("unroll")
+        for (int q = 0; q < K; ++q) zero |= prod[q] == 0.0;
+        if (__ballot(zero)) {
             const uint64_t at = hit_base(at0, kk);
-            double y = opaque_vgpr(v0);
+            double y[K];
+            _Pragma("unroll")
+            for (int q = 0; q < K; ++q) y[q] = opaque_vgpr(v0[q]);
             uint64_t sy = s0;
             for (int j = 0; j < 16; ++j) {
                 step(y, sy);
-                const uint64_t mk = __ballot(y == 0.0);
-                if (mk) record_hits_mask(J, at + (uint32_t)j, p, (mk >> __lane_id()) & 1, mk);
+                _Pragma("unroll")
+                for (int q = 0; q < K; ++q) {
+                    const uint64_t mk = __ballot(y[q] == 0.0);
+                    if (mk) record_hits_mask(J, at + (uint32_t)j, p[q], (mk >> __lane_id()) & 1, mk);
+                }
             }
         }
     }
     // the rest: blocks of 4 with one scalar branch, then single steps
     for (; kk + 4 <= cnt; kk += 4) {
-        step(v, s);
-        const uint64_t m0 = __ballot(v == 0.0);
-        step(v, s);
-        const uint64_t m1 = __ballot(v == 0.0);
-        step(v, s);
-        const uint64_t m2 = __ballot(v == 0.0);
-        step(v, s);
-        const uint64_t m3 = __ballot(v == 0.0);
-        if (m0 | m1 | m2 | m3) {
+        uint64_t mk[4][K];
+        uint64_t any = 0;
+        _Pragma("unroll")
+        for (int j = 0; j < 4; ++j) {
+            step(v, s);
+            _Pragma("unroll")
+            for (int q = 0; q < K; ++q) {
+                mk[j][q] = __ballot(v[q] == 0.0);
+                any |= mk[j][q];
+            }
+        }
+        if (any) {
             const uint32_t lane = __lane_id();
             const uint64_t at = hit_base(at0, kk);
-            if (m0) record_hits_mask(J, at, p, (m0 >> lane) & 1, m0);
-            if (m1) record_hits_mask(J, at + 1, p, (m1 >> lane) & 1, m1);
-            if (m2) record_hits_mask(J, at + 2, p, (m2 >> lane) & 1, m2);
-            if (m3) record_hits_mask(J, at + 3, p, (m3 >> lane) & 1, m3);
+            _Pragma("unroll")
+            for (int j = 0; j < 4; ++j) {
+                _Pragma("unroll")
+                for (int q = 0; q < K; ++q)
+                    if (mk[j][q]) record_hits_mask(J, at + (uint32_t)j, p[q], (mk[j][q] >> lane) & 1, mk[j][q]);
+            }
         }
     }
     for (; kk < cnt; ++kk) {
         step(v, s);
-        const uint64_t mk = __ballot(v == 0.0);
-        if (mk) record_hits_mask(J, hit_base(at0, kk), p, (mk >> __lane_id()) & 1, mk);
+        _Pragma("unroll")
+        for (int q = 0; q < K; ++q) {
+            const uint64_t mk = __ballot(v[q] == 0.0);
+            if (mk) record_hits_mask(J, hit_base(at0, kk), p[q], (mk >> __lane_id()) & 1, mk);
+        }
+    }
+}
+
+// The Smarandache FP64 form of Fast<SMARANDACHE>::run for K primes per lane
+// (every prime of the wave below FP_PMAX and above 8 n1); fin receives the
+// plain residues S(ne-1) mod p for the test hook.  The K primes share the
+// segment choices (each form's bound must hold for all of them), the scalar
+// sequences and the loop control.
+//
+    // r = S(n); step r = r m + c with m = b^digits(n+1) < 2^21 (host flag
+    // JOB_SM_NARROW) and c = n+1 < 2^32.  Write p = ph + pl with ph the top
+    // 32 significant bits of p and let k = round(r m / p), |k| < 2^21:
+    //   r' = (r m - k ph) + c - k pl.
+    // k ph is exact (at most 21 + 32 significant bits); r m - k ph =
+    // (r m - k p) + k pl lies below 2^53, so one fma computes it exactly;
+    // the rest is small.  |r m - k p| <= (1/2 + 2^-31) p, so |r'| <= 0.75 p
+    // whenever p > 8 n1 (checked per wave), and S(n) = 0 mod p <=> r == 0.
+    // c is wave-uniform and lives in scalar registers as the bit pattern
+    // of the double: inside one binade [2^e, 2^(e+1)) consecutive integers
+    // differ by 2^(52-e) in the pattern (and the carry into the exponent at
+    // 2^(e+1) still gives the exact value), so an integer add steps it and
+    // the VALU does 7 instructions per index, the compare included.
+    //
+    // The short form runs where its bound holds, the shifted form below
+    // elsewhere within the exact-product bound; the exact-product form
+    // runs in its place when it is switched off (JOB_SM_NO_SHIFT, a test
+    // hook) or where it does not apply (index 0, a prime dividing b).
+    //
+    // Short form, for a segment where every lane has p (0.75 m + 1) < 2^53:
+    // then |k p| < 2^53, so w = fma(k + M, -p, M p) = -k p is exact (M the
+    // rounding constant, M p exact for p < 2^50), r m + w = r m - k p is
+    // exact, and r' = (r m - k p) + c takes 4 instructions and the compare
+    // (the same bound |r'| <= 0.75 p holds).
+    //
+    // Exact-product form, for a segment where every lane has
+    // ((1/2 + 2^-30) p + n1) o < 2^53 with o the odd part of m (m = o 2^a).
+    // Every step of every form leaves |r| <= (1/2 + 2^-30) p + n1 (the
+    // reduced part, plus c < n1; the initial state is reduced), so
+    // |r o| < 2^53: r o is exact and so is r m = (r o) 2^a (a power-of-two
+    // scaling).  Then fma(-k, p, r m) has the exact value r m - k p,
+    // |.| <= (1/2 + 2^-30) p < 2^53, so it is exact, and
+    // r' = (r m - k p) + c takes 5 instructions and the compare.
+    // Power-of-two bases (o = 1) always qualify.
+template <int K>
+__device__ void run_fp_sm(const JobArgs &J, const uint64_t (&p)[K], uint64_t ns, uint64_t ne, uint64_t idx,
+                          uint64_t (&fin)[K])
+{
+    const uint64_t bb = J.param;
+    Fmod F[K];
+    double r[K], ph[K], pl[K], mp[K], pM[K], t[K];
+    _Pragma("unroll")
+    for (int q = 0; q < K; ++q) {
+        F[q].init(p[q]);
+        r[q] = sm_value_fp(F[q], bb, ns);
+        record_hits(J, idx, p[q], r[q] == 0.0);
+        const uint32_t L = 64 - __clzll(p[q]);
+        const uint64_t phi = L > 32 ? p[q] & ~((1ull << (L - 32)) - 1) : p[q];
+        ph[q] = (double)phi;
+        pl[q] = (double)(p[q] - phi);
+        pM[q] = F[q].p * Fmod::MAGIC;
+        t[q] = 0.0;
+    }
+    uint32_t d;
+    uint64_t nxt;
+    digits_pow(bb, ns + 1, d, nxt);
+    const double ob = (double)(bb >> __builtin_ctzll(bb)); // odd part of the base
+    double m = 1.0, om = 1.0;                                // m = bb^d, om its odd part
+    for (uint32_t i = 0; i < d; ++i) {
+        m *= (double)bb;
+        om *= ob;
+    }
+    _Pragma("unroll")
+    for (int q = 0; q < K; ++q) mp[q] = m * F[q].invp;
+    const double n1d = (double)J.n1; // c = n + 1 < n1 (n1 <= 2^32: JOB_SM_NARROW)
+    double inv_om = 0.0, inv_om_of = 0.0; // 1/om, and the om it belongs to
+    uint64_t n = ns; // index of the current state
+    // The shifted form holds t = r - n instead of r (`shifted`); index n
+    // itself is then not tested yet: its test is the next step's, or the
+    // one that follows restoring r.
+    bool shifted = false;
+    while (n + 1 < ne) {
+        // steps n -> n+1 .. seg_last -> seg_last+1 share m = b^digits(n+1)
+        // (digits grow when n+1 reaches nxt); each form also keeps its
+        // scalar sequence inside one binade
+        uint64_t seg_last = nxt - 2 < ne - 2 ? nxt - 2 : ne - 2;
+        // one wave-uniform choice per segment (m is fixed inside it)
+        const bool free_form = !(J.flags & JOB_SM_FULL_STEP);
+        // the short step, where it is exact, runs as fast as the shifted
+        // one and needs no state change (measured: 1-2 % faster)
+        bool short_form = free_form && !(J.flags & JOB_SM_NO_SHORT);
+        if (short_form) {
+            bool out = false;
+            _Pragma("unroll")
+            for (int q = 0; q < K; ++q) out |= !(F[q].p * (0.75 * m + 1.0) < SM_SHORT_LIMIT);
+            short_form = __ballot(out) == 0;
+        }
+        bool exact_ok = !short_form && free_form;
+        if (exact_ok) {
+            bool out = false;
+            _Pragma("unroll")
+            for (int q = 0; q < K; ++q) out |= !((F[q].p * SM_EXACT_HALF + n1d) * om < SM_SHORT_LIMIT);
+            exact_ok = __ballot(out) == 0;
+        }
+        bool shift_ok = exact_ok && n != 0 && !(J.flags & JOB_SM_NO_SHIFT);
+        if (shift_ok) {
+            bool small = false;
+            _Pragma("unroll")
+            for (int q = 0; q < K; ++q) small |= p[q] <= bb;
+            shift_ok = __ballot(small) == 0;
+        }
+        if (shift_ok) {
+            // Shifted form: the state is t = r - n (n the last appended
+            // number), so one fma forms r m = (t + n) m = t m + (n m) with the
+            // wave-uniform n m from the scalar unit, exactly (the exact-product
+            // bound: |r| o < 2^53 and n o < 2^53).  t' = r m - k p is exact
+            // (|.| <= (1/2 + 2^-30) p) and equals S(n+1) - (n+1) mod p.  The
+            // test of index n needs no appended number: S(n) = 0 mod p <=>
+            // S(n) m = 0 <=> t' == 0 (m = b^d is prime to p > b).  So a step
+            // is 4 instructions and the test, and the tests lag one index:
+            // the first step after entering the form repeats the test of
+            // index n (skipped), and the last index is tested when r = t + n
+            // is restored.  n m is stepped as a bit pattern inside the
+            // binade of n o (m = o 2^a), so a segment also ends where n o
+            // leaves its binade; t carries over to the next segment.
+            if (inv_om_of != om) { // 1/o, once per digit count
+                inv_om = 1.0 / om;
+                inv_om_of = om;
+            }
+            const uint64_t on = scalar_u64((uint64_t)om);
+            const uint32_t e2 = 63 - __clzll(n * on);
+            const uint64_t last_i = binade_last(e2, on, inv_om);
+            if (last_i < seg_last) seg_last = last_i;
+            const uint32_t cs = (uint32_t)(seg_last - n + 1);
+            const double mv = opaque_vgpr(m);
+            // n < n1 <= 2^32; the pattern is made scalar (m lives in a vector register)
+            uint64_t cmb = scalar_u64(__double_as_longlong((double)(uint32_t)n * m));
+            const uint64_t dcm = scalar_u64(on << (52 - e2));
+            auto step = [&](double (&v)[K], uint64_t &s) {
+                _Pragma("unroll")
+                for (int q = 0; q < K; ++q) {
+                    const double rm = fma(v[q], mv, __longlong_as_double(s));
+                    const double k = fma(rm, F[q].invp, Fmod::MAGIC) - Fmod::MAGIC;
+                    v[q] = fma(-k, F[q].p, rm);
+                }
+                s += dcm;
+            };
+            // entering the form: t = r - n, and the first step repeats the
+            // test of index n (recorded already), so it runs untested
+            const uint32_t done = shifted ? 0u : 1u;
+            if (done) {
+                _Pragma("unroll")
+                for (int q = 0; q < K; ++q) t[q] = r[q] - (double)(uint32_t)n;
+                step(t, cmb);
+                shifted = true;
+            }
+            // (wave-uniform values; the compiler cannot prove it for `shifted`)
+            cmb = scalar_u64(cmb);
+            sm_blocks<K, true>(J, p, scalar_u64(idx + done), __builtin_amdgcn_readfirstlane(cs - done), t, cmb,
+                               step);
+            n = seg_last + 1;
+            idx += cs;
+        } else {
+            if (shifted) { // back to r; index n is tested here
+                _Pragma("unroll")
+                for (int q = 0; q < K; ++q) {
+                    r[q] = t[q] + (double)(uint32_t)n;
+                    record_hits(J, idx, p[q], r[q] == 0.0);
+                }
+                shifted = false;
+            }
+            // the appended number c = n+1 is stepped inside its binade
+            const uint64_t c0 = n + 1;
+            const uint32_t e = 63 - __clzll(c0);
+            if ((2ull << e) - 2 < seg_last) seg_last = (2ull << e) - 2;
+            const uint32_t cnt = (uint32_t)(seg_last - n + 1); // below 2^32: n1 <= 2^32
+            const uint64_t delta = 1ull << (52 - e);
+            uint64_t cb = __double_as_longlong((double)c0);
+            if (short_form) {
+                sm_blocks<K, false>(J, p, idx + 1, cnt, r, cb, [&](double (&v)[K], uint64_t &s) {
+                    _Pragma("unroll")
+                    for (int q = 0; q < K; ++q) {
+                        const double k1 = fma(v[q], mp[q], Fmod::MAGIC);
+                        v[q] = fma(v[q], m, fma(k1, -F[q].p, pM[q])) + __longlong_as_double(s);
+                    }
+                    s += delta;
+                });
+            } else if (exact_ok) {
+                sm_blocks<K, false>(J, p, idx + 1, cnt, r, cb, [&](double (&v)[K], uint64_t &s) {
+                    _Pragma("unroll")
+                    for (int q = 0; q < K; ++q) {
+                        const double rm = v[q] * m;
+                        const double k = fma(v[q], mp[q], Fmod::MAGIC) - Fmod::MAGIC;
+                        v[q] = fma(-k, F[q].p, rm) + __longlong_as_double(s);
+                    }
+                    s += delta;
+                });
+            } else {
+                sm_blocks<K, false>(J, p, idx + 1, cnt, r, cb, [&](double (&v)[K], uint64_t &s) {
+                    _Pragma("unroll")
+                    for (int q = 0; q < K; ++q) {
+                        const double k = fma(v[q], mp[q], Fmod::MAGIC) - Fmod::MAGIC;
+                        const double u = fma(v[q], m, -(k * ph[q]));
+                        v[q] = fma(-k, pl[q], u + __longlong_as_double(s));
+                    }
+                    s += delta;
+                });
+            }
+            idx += cnt;
+            n = seg_last + 1;
+        }
+        if (n + 1 == nxt) { // the next appended number has one more digit
+            m *= (double)bb;
+            om *= ob;
+            _Pragma("unroll")
+            for (int q = 0; q < K; ++q) mp[q] = m * F[q].invp;
+            nxt = sat_mul(nxt, bb);
+        }
+    }
+    _Pragma("unroll")
+    for (int q = 0; q < K; ++q) {
+        if (shifted) {
+            r[q] = t[q] + (double)(uint32_t)n;
+            record_hits(J, idx, p[q], r[q] == 0.0);
+        }
+        fin[q] = F[q].canon(r[q], p[q]);
     }
 }
 
@@ -803,172 +1049,10 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
             { const double t = F.red(a + b + c); a = b; b = c; c = t; }, a == 0.0)
         return F.canon(a, p);
     } else if constexpr (FAM == FAM_SMARANDACHE) {
-        // r = S(n); step r = r m + c with m = b^digits(n+1) < 2^21 (host flag
-        // JOB_SM_NARROW) and c = n+1 < 2^32.  Write p = ph + pl with ph the top
-        // 32 significant bits of p and let k = round(r m / p), |k| < 2^21:
-        //   r' = (r m - k ph) + c - k pl.
-        // k ph is exact (at most 21 + 32 significant bits); r m - k ph =
-        // (r m - k p) + k pl lies below 2^53, so one fma computes it exactly;
-        // the rest is small.  |r m - k p| <= (1/2 + 2^-31) p, so |r'| <= 0.75 p
-        // whenever p > 8 n1 (checked per wave), and S(n) = 0 mod p <=> r == 0.
-        // c is wave-uniform and lives in scalar registers as the bit pattern
-        // of the double: inside one binade [2^e, 2^(e+1)) consecutive integers
-        // differ by 2^(52-e) in the pattern (and the carry into the exponent at
-        // 2^(e+1) still gives the exact value), so an integer add steps it and
-        // the VALU does 7 instructions per index, the compare included.
-        //
-        // The short form runs where its bound holds, the shifted form below
-        // elsewhere within the exact-product bound; the exact-product form
-        // runs in its place when it is switched off (JOB_SM_NO_SHIFT, a test
-        // hook) or where it does not apply (index 0, a prime dividing b).
-        //
-        // Short form, for a segment where every lane has p (0.75 m + 1) < 2^53:
-        // then |k p| < 2^53, so w = fma(k + M, -p, M p) = -k p is exact (M the
-        // rounding constant, M p exact for p < 2^50), r m + w = r m - k p is
-        // exact, and r' = (r m - k p) + c takes 4 instructions and the compare
-        // (the same bound |r'| <= 0.75 p holds).
-        //
-        // Exact-product form, for a segment where every lane has
-        // ((1/2 + 2^-30) p + n1) o < 2^53 with o the odd part of m (m = o 2^a).
-        // Every step of every form leaves |r| <= (1/2 + 2^-30) p + n1 (the
-        // reduced part, plus c < n1; the initial state is reduced), so
-        // |r o| < 2^53: r o is exact and so is r m = (r o) 2^a (a power-of-two
-        // scaling).  Then fma(-k, p, r m) has the exact value r m - k p,
-        // |.| <= (1/2 + 2^-30) p < 2^53, so it is exact, and
-        // r' = (r m - k p) + c takes 5 instructions and the compare.
-        // Power-of-two bases (o = 1) always qualify.
-        const uint64_t bb = J.param;
-        double r = sm_value_fp(F, bb, ns);
-        record_hits(J, idx, p, r == 0.0);
-        const uint32_t L = 64 - __clzll(p);
-        const uint64_t phi = L > 32 ? p & ~((1ull << (L - 32)) - 1) : p;
-        const double ph = (double)phi, pl = (double)(p - phi);
-        uint32_t d;
-        uint64_t nxt;
-        digits_pow(bb, ns + 1, d, nxt);
-        const double ob = (double)(bb >> __builtin_ctzll(bb)); // odd part of the base
-        double m = 1.0, om = 1.0;                                // m = bb^d, om its odd part
-        for (uint32_t i = 0; i < d; ++i) {
-            m *= (double)bb;
-            om *= ob;
-        }
-        double mp = m * F.invp;
-        const double pM = F.p * Fmod::MAGIC;
-        const double n1d = (double)J.n1; // c = n + 1 < n1 (n1 <= 2^32: JOB_SM_NARROW)
-        double inv_om = 0.0, inv_om_of = 0.0; // 1/om, and the om it belongs to
-        uint64_t n = ns; // index of the current state
-        // The shifted form holds t = r - n instead of r (`shifted`); index n
-        // itself is then not tested yet: its test is the next step's, or the
-        // one that follows restoring r.
-        bool shifted = false;
-        double t = 0.0;
-        while (n + 1 < ne) {
-            // steps n -> n+1 .. seg_last -> seg_last+1 share m = b^digits(n+1)
-            // (digits grow when n+1 reaches nxt); each form also keeps its
-            // scalar sequence inside one binade
-            uint64_t seg_last = nxt - 2 < ne - 2 ? nxt - 2 : ne - 2;
-            // one wave-uniform choice per segment (m is fixed inside it)
-            const bool free_form = !(J.flags & JOB_SM_FULL_STEP);
-            // the short step, where it is exact, runs as fast as the shifted
-            // one and needs no state change (measured: 1-2 % faster)
-            const bool short_form = free_form && !(J.flags & JOB_SM_NO_SHORT) &&
-                                    __ballot(!(F.p * (0.75 * m + 1.0) < SM_SHORT_LIMIT)) == 0;
-            const bool exact_ok = !short_form && free_form &&
-                                  __ballot(!((F.p * SM_EXACT_HALF + n1d) * om < SM_SHORT_LIMIT)) == 0;
-            if (exact_ok && n != 0 && !(J.flags & JOB_SM_NO_SHIFT) && __ballot(p <= bb) == 0) {
-                // Shifted form: the state is t = r - n (n the last appended
-                // number), so one fma forms r m = (t + n) m = t m + (n m) with the
-                // wave-uniform n m from the scalar unit, exactly (the exact-product
-                // bound: |r| o < 2^53 and n o < 2^53).  t' = r m - k p is exact
-                // (|.| <= (1/2 + 2^-30) p) and equals S(n+1) - (n+1) mod p.  The
-                // test of index n needs no appended number: S(n) = 0 mod p <=>
-                // S(n) m = 0 <=> t' == 0 (m = b^d is prime to p > b).  So a step
-                // is 4 instructions and the test, and the tests lag one index:
-                // the first step after entering the form repeats the test of
-                // index n (skipped), and the last index is tested when r = t + n
-                // is restored.  n m is stepped as a bit pattern inside the
-                // binade of n o (m = o 2^a), so a segment also ends where n o
-                // leaves its binade; t carries over to the next segment.
-                if (inv_om_of != om) { // 1/o, once per digit count
-                    inv_om = 1.0 / om;
-                    inv_om_of = om;
-                }
-                const uint64_t on = scalar_u64((uint64_t)om);
-                const uint32_t e2 = 63 - __clzll(n * on);
-                const uint64_t last_i = binade_last(e2, on, inv_om);
-                if (last_i < seg_last) seg_last = last_i;
-                const uint32_t cs = (uint32_t)(seg_last - n + 1);
-                const double mv = opaque_vgpr(m);
-                // n < n1 <= 2^32; the pattern is made scalar (m lives in a vector register)
-                uint64_t cmb = scalar_u64(__double_as_longlong((double)(uint32_t)n * m));
-                const uint64_t dcm = on << (52 - e2);
-                auto step = [&](double &v, uint64_t &s) {
-                    const double rm = fma(v, mv, __longlong_as_double(s));
-                    const double k = fma(rm, F.invp, Fmod::MAGIC) - Fmod::MAGIC;
-                    v = fma(-k, F.p, rm);
-                    s += dcm;
-                };
-                // entering the form: t = r - n, and the first step repeats the
-                // test of index n (recorded already), so it runs untested
-                const uint32_t done = shifted ? 0u : 1u;
-                if (done) {
-                    t = r - (double)(uint32_t)n;
-                    step(t, cmb);
-                    shifted = true;
-                }
-                // (wave-uniform values; the compiler cannot prove it for `shifted`)
-                sm_blocks<true>(J, p, scalar_u64(idx + done), __builtin_amdgcn_readfirstlane(cs - done), t, cmb, step);
-                n = seg_last + 1;
-                idx += cs;
-            } else {
-                if (shifted) { // back to r; index n is tested here
-                    r = t + (double)(uint32_t)n;
-                    record_hits(J, idx, p, r == 0.0);
-                    shifted = false;
-                }
-                // the appended number c = n+1 is stepped inside its binade
-                const uint64_t c0 = n + 1;
-                const uint32_t e = 63 - __clzll(c0);
-                if ((2ull << e) - 2 < seg_last) seg_last = (2ull << e) - 2;
-                const uint32_t cnt = (uint32_t)(seg_last - n + 1); // below 2^32: n1 <= 2^32
-                const uint64_t delta = 1ull << (52 - e);
-                uint64_t cb = __double_as_longlong((double)c0);
-                if (short_form) {
-                    sm_blocks<false>(J, p, idx + 1, cnt, r, cb, [&](double &v, uint64_t &s) {
-                        const double k1 = fma(v, mp, Fmod::MAGIC);
-                        v = fma(v, m, fma(k1, -F.p, pM)) + __longlong_as_double(s);
-                        s += delta;
-                    });
-                } else if (exact_ok) {
-                    sm_blocks<false>(J, p, idx + 1, cnt, r, cb, [&](double &v, uint64_t &s) {
-                        const double rm = v * m;
-                        const double k = fma(v, mp, Fmod::MAGIC) - Fmod::MAGIC;
-                        v = fma(-k, F.p, rm) + __longlong_as_double(s);
-                        s += delta;
-                    });
-                } else {
-                    sm_blocks<false>(J, p, idx + 1, cnt, r, cb, [&](double &v, uint64_t &s) {
-                        const double k = fma(v, mp, Fmod::MAGIC) - Fmod::MAGIC;
-                        const double t = fma(v, m, -(k * ph));
-                        v = fma(-k, pl, t + __longlong_as_double(s));
-                        s += delta;
-                    });
-                }
-                idx += cnt;
-                n = seg_last + 1;
-            }
-            if (n + 1 == nxt) { // the next appended number has one more digit
-                m *= (double)bb;
-                om *= ob;
-                mp = m * F.invp;
-                nxt = sat_mul(nxt, bb);
-            }
-        }
-        if (shifted) {
-            r = t + (double)(uint32_t)n;
-            record_hits(J, idx, p, r == 0.0);
-        }
-        return F.canon(r, p);
+        const uint64_t pp[1] = {p};
+        uint64_t fin[1];
+        run_fp_sm<1>(J, pp, ns, ne, idx, fin);
+        return fin[0];
     } else {
         return 0;
     }

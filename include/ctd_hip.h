@@ -91,6 +91,7 @@ typedef struct ctd_run_opts {
                                    exact-product step; else the general step); test hook */
 #define CTD_RUN_SM_NO_SHIFT 32u /* Smarandache FP64 form: no shifted step (the short, exact-product or
                                    general step, as CTD_RUN_SM_NO_SHORT allows); test hook */
+#define CTD_RUN_SM_NO_PAIR 64u  /* Smarandache: one prime per lane instead of two interleaved; test hook */
 
 typedef struct ctd_hip_engine ctd_hip_engine;
 

@@ -415,8 +415,9 @@ def test_smarandache_step_forms_with_hits(engine, base):
     r = o.trial_factor(o.SMARANDACHE, base, n0, n1, f0, f1, threads=16)
     assert sum(len(x) for x in r) > 100
     for flags in SM_FORM_FLAGS:
-        g, _, _ = gpu_lists(engine, o.SMARANDACHE, base, n0, n1, f0, f1, flags=flags)
-        assert g == r, flags
+        for pair in (0, nat.CTD_RUN_SM_NO_PAIR):
+            g, _, _ = gpu_lists(engine, o.SMARANDACHE, base, n0, n1, f0, f1, flags=flags | pair)
+            assert g == r, flags | pair
 
 
 @pytest.mark.parametrize("base", [10, 3, 7, 2, 16, 12])
@@ -437,7 +438,7 @@ def test_smarandache_shifted_step_segment_edges(engine, base):
         ranges |= {(max(1, h - 1 - j), h + 19) for j in (0, 1, 7, 14, 15, 16)}
     for n0, n1 in sorted(ranges):
         want = o.trial_factor(o.SMARANDACHE, base, n0, n1, f0, f1)
-        for flags in SM_FORM_FLAGS:
+        for flags in SM_FORM_FLAGS + (nat.CTD_RUN_SM_NO_PAIR, nat.CTD_RUN_SM_NO_PAIR | nat.CTD_RUN_SM_NO_SHORT):
             g, _, _ = gpu_lists(engine, o.SMARANDACHE, base, n0, n1, f0, f1, flags=flags)
             assert g == want, (n0, n1, flags)
 
@@ -448,7 +449,7 @@ def test_smarandache_prime_dividing_the_base(engine):
     # take the other steps.  p > 8 n1 keeps the FP64 form.
     for n0, n1 in ((1, 5), (2, 7), (3, 4)):
         want = o.trial_factor(o.SMARANDACHE, 61, n0, n1, 41, 4000)
-        for flags in SM_FORM_FLAGS:
+        for flags in SM_FORM_FLAGS + (nat.CTD_RUN_SM_NO_PAIR | nat.CTD_RUN_SM_NO_SHORT,):
             g, _, _ = gpu_lists(engine, o.SMARANDACHE, 61, n0, n1, 41, 4000, flags=flags)
             assert g == want, (n0, n1, flags)
 

@@ -84,8 +84,9 @@ def test_smarandache_base10_digit_boundaries(engine):
     primes = random_primes(1 << 32, 1 << 34, 256, seed=1)
     for n0, n1 in ((1, 120), (95, 1005), (9990, 10011), (99990, 100012), (1, 20000)):
         want = o.residues_last(o.SMARANDACHE, 10, n0, n1 - n0, primes)
-        got = engine.residues(o.SMARANDACHE, 10, n0, n1, primes, 0)
-        assert np.array_equal(got, want), (n0, n1)
+        for flags in (0, nat.CTD_RUN_SM_NO_PAIR):
+            got = engine.residues(o.SMARANDACHE, 10, n0, n1, primes, flags)
+            assert np.array_equal(got, want), (n0, n1, flags)
 
 
 def test_residues_rejects_even_moduli(engine):
@@ -137,10 +138,25 @@ def test_smarandache_short_step_limit(engine, base):
         mixed = [x for pair in zip(below[64:], far) for x in pair] if far else below[64:]
         primes = below[:64] + above + mixed
         want = o.residues_last(o.SMARANDACHE, base, n0, n1 - n0, primes)
-        for flags in (0, nat.CTD_RUN_SM_NO_SHORT, nat.CTD_RUN_SM_NO_SHIFT, nat.CTD_RUN_SM_FULL_STEP):
-            got = engine.residues(o.SMARANDACHE, base, n0, n1, primes, flags)
-            bad = np.nonzero(got != want)[0]
-            assert len(bad) == 0, (m, flags, [(primes[i], int(got[i]), int(want[i])) for i in bad[:5]])
+        _check_sm_forms(engine, base, n0, n1, primes, want, m,
+                        (0, nat.CTD_RUN_SM_NO_SHORT, nat.CTD_RUN_SM_NO_SHIFT, nat.CTD_RUN_SM_FULL_STEP))
+
+
+def _check_sm_forms(engine, base, n0, n1, primes, want, tag, flag_sets):
+    """Final residues of every 64-prime wave of `primes` in each step form,
+    one prime per lane (CTD_RUN_SM_NO_PAIR) and two: the two-prime kernel
+    pairs the waves 2i and 2i+1 of its input, so there every wave is given
+    twice in a row and a pair holds the same primes (its step choices then
+    follow that wave alone)."""
+    waves = [primes[i:i + 64] for i in range(0, len(primes), 64)]
+    doubled = [x for w in waves for x in w + w]
+    ref = dict(zip(primes, (int(x) for x in want)))
+    want2 = np.array([ref[x] for x in doubled], dtype=np.uint64)
+    for flags in flag_sets:
+        for ps, exp, fl in ((primes, want, flags | nat.CTD_RUN_SM_NO_PAIR), (doubled, want2, flags)):
+            got = engine.residues(o.SMARANDACHE, base, n0, n1, ps, fl)
+            bad = np.nonzero(got != exp)[0]
+            assert len(bad) == 0, (tag, fl, [(ps[i], int(got[i]), int(exp[i])) for i in bad[:5]])
 
 
 def _odd_part(m):
@@ -196,11 +212,9 @@ def test_smarandache_exact_product_limit(engine, base):
     assert sets
     for m, primes in sets:
         want = o.residues_last(o.SMARANDACHE, base, n0, n1 - n0, primes)
-        for flags in (0, nat.CTD_RUN_SM_NO_SHORT, nat.CTD_RUN_SM_NO_SHIFT,
-                      nat.CTD_RUN_SM_NO_SHIFT | nat.CTD_RUN_SM_NO_SHORT, nat.CTD_RUN_SM_FULL_STEP):
-            got = engine.residues(o.SMARANDACHE, base, n0, n1, primes, flags)
-            bad = np.nonzero(got != want)[0]
-            assert len(bad) == 0, (m, flags, [(primes[i], int(got[i]), int(want[i])) for i in bad[:5]])
+        _check_sm_forms(engine, base, n0, n1, primes, want, m,
+                        (0, nat.CTD_RUN_SM_NO_SHORT, nat.CTD_RUN_SM_NO_SHIFT,
+                         nat.CTD_RUN_SM_NO_SHIFT | nat.CTD_RUN_SM_NO_SHORT, nat.CTD_RUN_SM_FULL_STEP))
 
 
 def test_tribonacci_lazy_form_limit(engine):
