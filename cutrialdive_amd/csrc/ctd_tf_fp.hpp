@@ -251,12 +251,17 @@ __device__ inline Rphi rphi_theta_pow(const Fmod &F, uint64_t e)
     return x;
 }
 
-// Indices per reduction of the theta stride form (64 compares of 32-bit
-// target words against 33 VALU instructions of the ring product; the 64
-// words take 64 registers, double targets would take 128 and leave 3 waves
-// per SIMD), and the shortest work item that takes it (its set-up is about 60
-// ring products).
-constexpr int THETA_STRIDE = 64;
+// Indices per reduction of the theta stride form: THETA_STRIDE compares of
+// 32-bit target words against 33 VALU instructions of the ring product, so
+// 1 + 33 / THETA_STRIDE per index.  Each target word takes a register: with
+// 128 words the Lucas kernel needs 166 registers (3 waves per SIMD) and the
+// Fibonacci kernel 172 (2 waves, which it runs 9 % slower than 3); 112 keeps
+// Fibonacci at 156 (3 waves).  Measured on the mixed batch's jobs against 64
+// words (106 registers, 4 waves): Lucas 14 % faster with 128, Fibonacci 10 %
+// with 112.  THETA_MIN_STEPS is the shortest work item that takes the form
+// (its set-up is about 60 ring products and the targets).
+template <uint32_t FAM>
+constexpr int THETA_STRIDE = FAM == FAM_LUCAS ? 128 : 112;
 constexpr uint64_t THETA_MIN_STEPS = 1024;
 // primes below which the Tribonacci lazy form keeps 8 unreduced sums exact
 constexpr uint64_t TRIB_LAZY_PMAX = 1ull << 44;
@@ -890,13 +895,13 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
     } else if constexpr (FAM == FAM_FIBONACCI || FAM == FAM_LUCAS) {
         if (J.n0 != 0 && steps >= THETA_MIN_STEPS && __ballot(p == 5) == 0) {
             // theta stride form: x = theta^n, hit when x == eps (eps = 1 for
-            // Fibonacci, -1 for Lucas).  The THETA_STRIDE indices after x_n
+            // Fibonacci, -1 for Lucas).  The THETA_STRIDE<FAM> indices after x_n
             // are hits when x_n == eps theta^-j.  Every power of theta has
             // norm u^2 + uv - v^2 = 1, so for a given v at most two u qualify
             // (u and -v - u): the loop compares (a 32-bit word of) the v
             // coordinate only, one compare per index, and a match is
-            // confirmed on both coordinates before it is recorded.  x advances THETA_STRIDE indices by one
-            // ring product with theta^THETA_STRIDE.
+            // confirmed on both coordinates before it is recorded.  x advances THETA_STRIDE<FAM> indices by one
+            // ring product with theta^THETA_STRIDE<FAM>.
             const double eps = FAM == FAM_FIBONACCI ? 1.0 : -1.0;
             Rphi x = rphi_theta_pow(F, ns);
             record_hits(J, idx, p, x.u == eps && x.v == 0.0);
@@ -904,18 +909,18 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
             // moves an integer |v| < 2^51 into the low bits of the pattern).
             // Equal v give equal words; a word match that is not a hit (values
             // 2^32 apart) is rejected by the exact replay.  Two words share a
-            // 64-bit register pair, so the targets take THETA_STRIDE registers.
+            // 64-bit register pair, so the targets take THETA_STRIDE<FAM> registers.
             // The v coordinates of eps theta^-j follow v_(j+1) = -3 v_j - v_(j-1)
             // from v_0 = 0, v_1 = eps: theta^-1 = phi - 2 is a root of
             // y^2 + 3y + 1 (theta + theta^-1 = -(phi^2 + psi^2) = -L(2) = -3),
             // so every coordinate of eps theta^-j obeys that recurrence.  The
             // sum stays below 2p + 1, so red gives the same (symmetric)
             // representative as the ring product does for x.
-            uint64_t Tw[THETA_STRIDE / 2];
+            uint64_t Tw[THETA_STRIDE<FAM> / 2];
             {
                 double va = 0.0, vb = eps; // (v_j, v_(j+1))
                 _Pragma("unroll")
-                for (int j = 0; j < THETA_STRIDE; j += 2) {
+                for (int j = 0; j < THETA_STRIDE<FAM>; j += 2) {
                     const uint64_t w0 = low_word(vb);
                     va = F.red(fma(-3.0, vb, -va));
                     const uint64_t w1 = low_word(va);
@@ -925,23 +930,22 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
                     Tw[j / 2] = pair;
                 }
             }
-            Rphi g = {-1.0, -1.0}; // theta^THETA_STRIDE by squaring
-            for (int j = 1; j < THETA_STRIDE; j <<= 1) g = rphi_mul(F, g, g);
+            const Rphi g = rphi_theta_pow(F, THETA_STRIDE<FAM>);
             const double gw = F.red(g.u + g.v);
             const uint32_t steps32 = (uint32_t)steps;
             uint32_t k = 0;
-            for (; k + THETA_STRIDE <= steps32; k += THETA_STRIDE) {
+            for (; k + THETA_STRIDE<FAM> <= steps32; k += THETA_STRIDE<FAM>) {
                 const uint32_t xw = low_word(x.v);
                 uint64_t any = 0;
                 _Pragma("unroll")
-                for (int q = 0; q < THETA_STRIDE / 16; ++q) any |= words_match16(xw, Tw + 8 * q);
+                for (int q = 0; q < THETA_STRIDE<FAM> / 16; ++q) any |= words_match16(xw, Tw + 8 * q);
                 if (any) {
                     // replay the block: x theta^(j+1) == eps on both coordinates
                     // (x is made opaque so that the loop's masks are not kept
                     // live for this rare path)
                     const uint64_t at = hit_base(idx, k);
                     Rphi y = {opaque_vgpr(x.u), opaque_vgpr(x.v)};
-                    for (int j = 0; j < THETA_STRIDE; ++j) {
+                    for (int j = 0; j < THETA_STRIDE<FAM>; ++j) {
                         y = rphi_mul_theta(F, y);
                         record_hits(J, at + 1 + j, p, y.u == eps && y.v == 0.0);
                     }
