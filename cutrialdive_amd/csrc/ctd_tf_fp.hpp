@@ -263,6 +263,8 @@ __device__ inline Rphi rphi_theta_pow(const Fmod &F, uint64_t e)
 template <uint32_t FAM>
 constexpr int THETA_STRIDE = FAM == FAM_LUCAS ? 128 : 112;
 constexpr uint64_t THETA_MIN_STEPS = 1024;
+// target words of the doubling families' word stride (primes from 2^37)
+constexpr int DOUBLING_WORDS = 128;
 // primes below which the Tribonacci lazy form keeps 8 unreduced sums exact
 constexpr uint64_t TRIB_LAZY_PMAX = 1ull << 44;
 
@@ -400,15 +402,16 @@ __device__ __forceinline__ double doubling_blocks(const JobArgs &J, uint64_t p, 
 }
 
 // nb blocks of B doubling indices.  Tw packs the words of the targets
-// t 2^-(j+1) (j < 64), two per register pair; x advances B indices by one
+// t 2^-(j+1) (j < 2 N), two per register pair; x advances B indices by one
 // exact product with C = 2^B mod p.  A block costs the word of x, B compares
 // and 9 instructions of product; a block with a word match is replayed one
 // index at a time, exactly.
-template <int B>
+template <int B, int N>
 __device__ __forceinline__ double doubling_words(const JobArgs &J, uint64_t p, uint64_t &idx, uint32_t nb,
-                                                 double x, const uint64_t (&Tw)[32], const Fmod &F, double C,
+                                                 double x, const uint64_t (&Tw)[N], const Fmod &F, double C,
                                                  double tgt)
 {
+    static_assert(B <= 2 * N, "a block needs one target per index");
     for (uint32_t i = 0; i < nb; ++i) {
         const uint32_t xw = low_word(x);
         uint64_t any = 0;
@@ -826,12 +829,14 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
             left %= 16;
             }
         } else {
-            // targets t 2^-(j+1), j < 64, canonical, as packed words
-            uint64_t Tw[32];
+            // targets t 2^-(j+1), j < DOUBLING_WORDS, canonical, as packed
+            // words (one register each; 128 words leave 3 waves per SIMD,
+            // which the loop runs at full rate)
+            uint64_t Tw[DOUBLING_WORDS / 2];
             {
                 double t = tgt;
                 _Pragma("unroll")
-                for (int j = 0; j < 64; j += 2) {
+                for (int j = 0; j < DOUBLING_WORDS; j += 2) {
                     t = half_canon(F, t);
                     const uint64_t w0 = low_word(t);
                     t = half_canon(F, t);
@@ -841,8 +846,11 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
                     Tw[j / 2] = pair;
                 }
             }
-            // 2^16 mod p by one exact reduction, 2^64 mod p by two squarings
+            // 2^16 mod p by one exact reduction, 2^64 and 2^128 mod p by squarings
             const double C16 = F.red(65536.0), C32 = mulc_canon(F, C16, C16), C64 = mulc_canon(F, C32, C32);
+            x = doubling_words<DOUBLING_WORDS>(J, p, idx, left / DOUBLING_WORDS, x, Tw, F,
+                                               mulc_canon(F, C64, C64), tgt);
+            left %= DOUBLING_WORDS;
             x = doubling_words<64>(J, p, idx, left / 64, x, Tw, F, C64, tgt);
             left %= 64;
             x = doubling_words<16>(J, p, idx, left / 16, x, Tw, F, C16, tgt);

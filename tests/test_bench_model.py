@@ -50,7 +50,7 @@ def test_smarandache_shifted_and_general_windows():
 
 def test_stride_forms_by_prime_size():
     t = bench.ops_table()
-    shift, words = 1.1875, 1.15625
+    shift, words = 1.1875, 138 / 128
     assert bench.window_ops(t, "proth", 1 << 30, 1 << 31) == shift
     assert bench.window_ops(t, "proth", 1 << 40, 1 << 41) == words
     assert bench.window_ops(t, "tribonacci", 1 << 40, 1 << 41) == 4.5
