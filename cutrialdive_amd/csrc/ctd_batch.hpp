@@ -150,7 +150,6 @@ JobArgs tile_args(const ctd_hip_engine *E, const JobPlan &P, uint64_t est_np, ui
     J.chunk = chunk;
     J.nchunks = (uint32_t)div_up(P.N, chunk);
     J.flags = (J.fam == FAM_SMARANDACHE && sm_narrow_ok(J.param, J.n1, pmax)) ? JOB_SM_NARROW : 0u;
-    if (pmax < (1ull << 37)) J.flags |= JOB_P37;
     if (pmax < FP_PMAX && P.fp64) J.flags |= JOB_FP64;
     J.flags |= P.stepped;
     return J;

@@ -75,7 +75,7 @@ __global__ void __launch_bounds__(256) tf_generic_kernel(JobArgs J, const uint64
 }
 
 // Fast kernel: primes below 2^63 with the family's narrow step (ctd_tf_fast.hpp).
-template <uint32_t FAM, bool SHIFT>
+template <uint32_t FAM>
 __global__ void __launch_bounds__(256) tf_fast_kernel(JobArgs J, const uint64_t *__restrict__ primes,
                                                        uint64_t nprimes, const unsigned long long *np_dev)
 {
@@ -92,7 +92,7 @@ __global__ void __launch_bounds__(256) tf_fast_kernel(JobArgs J, const uint64_t 
         const uint64_t ns = J.n0 + c * J.chunk;
         uint64_t ne = ns + J.chunk;
         if (ne > J.n1 || ne < ns) ne = J.n1;
-        Fast<FAM>::template run<SHIFT>(J, p, ns, ne, i);
+        Fast<FAM>::run(J, p, ns, ne, i);
     }
 }
 
@@ -517,13 +517,10 @@ template <uint32_t FAM>
 void launch_tf(const JobArgs &J, const uint64_t *primes, uint64_t nprimes, const unsigned long long *np_dev,
                bool fast, dim3 grid, dim3 block, hipStream_t st)
 {
-    constexpr bool doubling = FAM == FAM_MERSENNE || FAM == FAM_PROTH || FAM == FAM_RIESEL;
-    if (fast && doubling && (J.flags & JOB_P37))
-        hipLaunchKernelGGL((tf_fast_kernel<FAM, doubling>), grid, block, 0, st, J, primes, nprimes, np_dev);
-    else if (fast && FAM == FAM_SMARANDACHE && !(J.flags & JOB_SM_NO_PAIR))
+    if (fast && FAM == FAM_SMARANDACHE && !(J.flags & JOB_SM_NO_PAIR))
         hipLaunchKernelGGL(tf_sm_pair_kernel, grid, block, 0, st, J, primes, nprimes, np_dev);
     else if (fast)
-        hipLaunchKernelGGL((tf_fast_kernel<FAM, false>), grid, block, 0, st, J, primes, nprimes, np_dev);
+        hipLaunchKernelGGL(tf_fast_kernel<FAM>, grid, block, 0, st, J, primes, nprimes, np_dev);
     else
         hipLaunchKernelGGL(tf_generic_kernel<FAM>, grid, block, 0, st, J, primes, nprimes, np_dev);
 }
@@ -694,7 +691,6 @@ int run_job(ctd_hip_engine *E, const ctd_job *job, const ctd_run_opts *opts, con
                 if (opts && (opts->flags & CTD_RUN_SM_NO_SHORT)) J.flags |= JOB_SM_NO_SHORT;
                 if (opts && (opts->flags & CTD_RUN_SM_NO_SHIFT)) J.flags |= JOB_SM_NO_SHIFT;
                 if (opts && (opts->flags & CTD_RUN_SM_NO_PAIR)) J.flags |= JOB_SM_NO_PAIR;
-                if (bhi - 1 < (1ull << 37)) J.flags |= JOB_P37;
                 J.flags |= stepped;
                 const uint32_t block = (opts && opts->block_size > 0) ? (uint32_t)opts->block_size : 256u;
                 uint64_t items = ((np + 63) & ~63ull) * nch;
@@ -1239,7 +1235,6 @@ int ctd_hip_residues(ctd_hip_engine *E, const ctd_job *job, const uint64_t *prim
         if (flags & CTD_RUN_SM_NO_SHORT) J.flags |= JOB_SM_NO_SHORT;
         if (flags & CTD_RUN_SM_NO_SHIFT) J.flags |= JOB_SM_NO_SHIFT;
         if (flags & CTD_RUN_SM_NO_PAIR) J.flags |= JOB_SM_NO_PAIR;
-        if (pmax < (1ull << 37)) J.flags |= JOB_P37;
         if (flags & CTD_RUN_STEPPED) J.flags |= JOB_STEPPED;
         const uint64_t items = (nprimes + 63) & ~63ull;
         launch_tf_any(J, E->d_primes, nprimes, nullptr, fast, dim3((uint32_t)div_up(items, 256)), dim3(256),

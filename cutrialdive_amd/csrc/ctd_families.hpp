@@ -42,7 +42,6 @@ enum Family : uint32_t {
 constexpr uint32_t JOB_SM_NARROW = 1; // smarandache 32-bit-quotient step allowed (see ctd_tf_fast.hpp)
 constexpr uint32_t JOB_FP64 = 2;      // every prime below 2^48: the FP64 forms are allowed (ctd_tf_fp.hpp)
 constexpr uint32_t JOB_SM_FULL_STEP = 4; // smarandache FP64 form without its short step (ctd_tf_fp.hpp)
-constexpr uint32_t JOB_P37 = 8;       // every prime below 2^37: the doubling shift form (ctd_tf_fp.hpp)
 constexpr uint32_t JOB_STEPPED = 16;  // fixed-n families: step every index (no closed form, ctd_tf_fast.hpp)
 constexpr uint32_t JOB_SM_NO_SHORT = 32; // smarandache FP64 form: skip the short step (exact-product or general)
 constexpr uint32_t JOB_SM_NO_SHIFT = 64; // smarandache FP64 form: skip the shifted step (short, exact-product or general)

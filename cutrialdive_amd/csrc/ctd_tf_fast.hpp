@@ -284,9 +284,7 @@ template <uint32_t FAM>
 struct Fast {
     __host__ __device__ static bool fits(uint64_t pmax) { return pmax < FAST_PMAX; }
 
-    // ns, ne are wave-uniform; pi is the prime's index in the batch.  SHIFT:
-    // the launch holds primes below 2^37 only (JOB_P37), see run_fp.
-    template <bool SHIFT = false>
+    // ns, ne are wave-uniform; pi is the prime's index in the batch.
     __device__ static void run(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t ne, uint64_t pi)
     {
         // test hook: the plain residue S(ne-1) mod p
@@ -307,7 +305,7 @@ struct Fast {
             }
         }
         if (use_fp<FAM>(J, p, ne - ns - 1)) {
-            emit(run_fp<FAM, SHIFT>(J, p, ns, ne, ns - J.n0, ne - ns - 1));
+            emit(run_fp<FAM>(J, p, ns, ne, ns - J.n0, ne - ns - 1));
             return;
         }
         Mont M;

@@ -25,23 +25,20 @@
 // after the state x_n are tested by S compares of x_n against precomputed
 // targets, and the state advances S indices at once:
 //   doubling  x_{n+j} = x_n 2^j:   hit at n+j  <=>  x_n == t 2^-j,
-//             p < 2^37: S = 16, x <- x 2^S - p round(x 2^S / p) (3 FMAs: the
-//             product by 2^S is exact, p round() is exact below 2^53, and
-//             |x 2^S / p| < 2^(S-1) keeps the quotient exact);
-//             above: S = 64, x <- x (2^S mod p), an exact product
-//             (mulc_canon, 9 instructions);
-//   fixed n   x_{n+j} = x_n + j c: hit at n+j  <=>  x_n == -j c,
+//             S = 128 (then 64 and 16 for the tail), x <- x (2^S mod p), an
+//             exact product (mulc_canon, 9 instructions);
+//   fixed n   x_{n+j} = x_n + j c: hit at n+j  <=>  x_n == -j c, S = 64,
 //             x <- red(x + S c)  (4 instructions).
-// With S = 64 the compares use the low 32-bit words of x + M and of the
-// targets (low_word), two target words per register pair, and a block whose
-// words match is replayed one index at a time with the exact value.
+// The compares use the low 32-bit words of x + M and of the targets
+// (low_word), two target words per register pair, and a block whose words
+// match is replayed one index at a time with the exact value.
 //
 // Per index this costs (VALU instructions, compare included):
 //
-//   doubling x = 2x           1.19 (p < 2^37), 1.16 (integer W64 form: 7)
+//   doubling x = 2x           1.08 (integer W64 form: 7)
 //   fixed n  x = x + c        1.08 (6) when stepped; by default the closed
 //                             form of ctd_tf_fast.hpp solves once per prime
-//   fibonacci / lucas         1.53 theta stride form, 5 pair form (6)
+//   fibonacci / lucas         1.30 / 1.27 theta stride form, 5 pair form (6)
 //   tribonacci                4.5 (p < 2^44), 5.5 (11.25)
 //   smarandache r m + n + 1   7   (12.25), for p > 8 n1; 5 when p (0.75 m + 1) < 2^53
 //                             (short step) or (p / 2 + n1) o < 2^53 (shifted step,
@@ -263,7 +260,7 @@ __device__ inline Rphi rphi_theta_pow(const Fmod &F, uint64_t e)
 template <uint32_t FAM>
 constexpr int THETA_STRIDE = FAM == FAM_LUCAS ? 128 : 112;
 constexpr uint64_t THETA_MIN_STEPS = 1024;
-// target words of the doubling families' word stride (primes from 2^37)
+// target words of the doubling families' word stride
 constexpr int DOUBLING_WORDS = 128;
 // primes below which the Tribonacci lazy form keeps 8 unreduced sums exact
 constexpr uint64_t TRIB_LAZY_PMAX = 1ull << 44;
@@ -335,70 +332,6 @@ __device__ __forceinline__ uint64_t words_match16(uint32_t w, const uint64_t *Tw
                    [te] "v"((uint32_t)Tw[7]), [tf] "v"((uint32_t)(Tw[7] >> 32))
                  : "vcc", "scc");
     return gone;
-}
-
-// The same for 16 exact FP64 targets.
-__device__ __forceinline__ uint64_t doubles_match16(double x, const double *T)
-{
-    uint64_t save, gone;
-    asm volatile("s_mov_b64 %[s], exec\n\t"
-                 "v_cmpx_neq_f64 vcc, %[x], %[t0]\n\tv_cmpx_neq_f64 vcc, %[x], %[t1]\n\t"
-                 "v_cmpx_neq_f64 vcc, %[x], %[t2]\n\tv_cmpx_neq_f64 vcc, %[x], %[t3]\n\t"
-                 "v_cmpx_neq_f64 vcc, %[x], %[t4]\n\tv_cmpx_neq_f64 vcc, %[x], %[t5]\n\t"
-                 "v_cmpx_neq_f64 vcc, %[x], %[t6]\n\tv_cmpx_neq_f64 vcc, %[x], %[t7]\n\t"
-                 "v_cmpx_neq_f64 vcc, %[x], %[t8]\n\tv_cmpx_neq_f64 vcc, %[x], %[t9]\n\t"
-                 "v_cmpx_neq_f64 vcc, %[x], %[ta]\n\tv_cmpx_neq_f64 vcc, %[x], %[tb]\n\t"
-                 "v_cmpx_neq_f64 vcc, %[x], %[tc]\n\tv_cmpx_neq_f64 vcc, %[x], %[td]\n\t"
-                 "v_cmpx_neq_f64 vcc, %[x], %[te]\n\tv_cmpx_neq_f64 vcc, %[x], %[tf]\n\t"
-                 "s_xor_b64 %[g], %[s], exec\n\t"
-                 "s_mov_b64 exec, %[s]"
-                 : [s] "=&s"(save), [g] "=&s"(gone)
-                 : [x] "v"(x), [t0] "v"(T[0]), [t1] "v"(T[1]), [t2] "v"(T[2]), [t3] "v"(T[3]), [t4] "v"(T[4]),
-                   [t5] "v"(T[5]), [t6] "v"(T[6]), [t7] "v"(T[7]), [t8] "v"(T[8]), [t9] "v"(T[9]), [ta] "v"(T[10]),
-                   [tb] "v"(T[11]), [tc] "v"(T[12]), [td] "v"(T[13]), [te] "v"(T[14]), [tf] "v"(T[15])
-                 : "vcc", "scc");
-    return gone;
-}
-
-// nb blocks of NS doubling strides of S indices for p < 2^(53-S) (x advances
-// 2^S per stride by an exact shift, T[j] = t 2^-(j+1)).  The scalar unit
-// issues the OR of every compare's wave mask and the loop control for the
-// four SIMDs of a CU, so with NS S + 3 NS VALU against about NS S + 5 scalar
-// instructions per block, blocks of 64 indices keep the loop VALU-bound.
-// Only the OR of the masks is kept: a block with a hit is replayed from its
-// first state to record each hit.  x 2^S - p round(x 2^S / p) is computed
-// with the subtracted multiple -(q p - M p) (q the biased quotient) as a
-// negated addend, so the FMA writes x's own register (no copy per stride).
-template <int S, int NS>
-__device__ __forceinline__ double doubling_blocks(const JobArgs &J, uint64_t p, uint64_t &idx, uint32_t nb,
-                                                  double x, const double (&T)[S], double ip, double c,
-                                                  double P, double pM)
-{
-    for (uint32_t i = 0; i < nb; ++i) {
-        const double x0 = x;
-        uint64_t any = 0;
-        _Pragma("unroll")
-        for (int h = 0; h < NS; ++h) {
-            static_assert(S == 16, "one 16-target compare block per stride");
-            any |= doubles_match16(x, T);
-            x = fma(x, c, -fma(fma(x, ip, Fmod::MAGIC), P, -pM));
-        }
-        if (any) {
-            const uint32_t lane = __lane_id();
-            const uint64_t at = hit_base(idx, i * (uint32_t)(S * NS));
-            double y = opaque_vgpr(x0); // no reuse of the loop's masks (they would stay live)
-            for (int h = 0; h < NS; ++h) {
-                _Pragma("unroll")
-                for (int j = 0; j < S; ++j) {
-                    const uint64_t m = __ballot(y == T[j]);
-                    if (m) record_hits_mask(J, at + 1 + (uint32_t)(h * S + j), p, (m >> lane) & 1, m);
-                }
-                y = fma(y, c, -fma(fma(y, ip, Fmod::MAGIC), P, -pM));
-            }
-        }
-    }
-    idx += (uint64_t)nb * (S * NS);
-    return x;
 }
 
 // nb blocks of B doubling indices.  Tw packs the words of the targets
@@ -797,7 +730,7 @@ __device__ void run_fp_sm(const JobArgs &J, const uint64_t (&p)[K], uint64_t ns,
 // The FP64 form of Fast<FAM>::run for a wave whose primes are all below
 // FP_PMAX (and, for smarandache, above 8 n1).  Returns the plain
 // residue S(ne-1) mod p for the test hook.
-template <uint32_t FAM, bool SHIFT>
+template <uint32_t FAM>
 __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t ne, uint64_t idx, uint64_t steps)
 {
     Fmod F;
@@ -810,25 +743,7 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
         const double tgt = (FAM == FAM_PROTH) ? -1.0 : 1.0;
         record_hits(J, idx, p, x == tgt);
         uint32_t left = (uint32_t)steps;
-        // The two forms are separate kernels (SHIFT, chosen by the host when
-        // every prime of the launch is below 2^37): sharing one kernel would
-        // give the shift form the word form's register count and cost it two
-        // waves per SIMD.
-        if constexpr (SHIFT) {
-            // exact shift by 2^16 (3 instructions per 16 indices); a wave with a
-            // prime above 2^37 (not launched by the host) takes single steps
-            if (__ballot(p >= (1ull << 37)) == 0) {
-            double T[16];
-            T[0] = half_canon(F, tgt);
-            _Pragma("unroll")
-            for (int j = 1; j < 16; ++j) T[j] = half_canon(F, T[j - 1]);
-            const double pM = F.p * Fmod::MAGIC, ip = F.invp * 65536.0, c = opaque_vgpr(65536.0);
-            x = doubling_blocks<16, 4>(J, p, idx, left / 64, x, T, ip, c, F.p, pM);
-            left %= 64;
-            x = doubling_blocks<16, 1>(J, p, idx, left / 16, x, T, ip, c, F.p, pM);
-            left %= 16;
-            }
-        } else {
+        {
             // targets t 2^-(j+1), j < DOUBLING_WORDS, canonical, as packed
             // words (one register each; 128 words leave 3 waves per SIMD,
             // which the loop runs at full rate)

@@ -2,8 +2,8 @@
 
 The model must pick, for each prime window, the loop form the kernels pick:
 the Smarandache short or shifted step (5 instructions) where (p / 2 + n1) o
-< 2^53 for the odd part o of the digit multiplier m, the shift stride below 2^37 and the word stride above for the
-doubling families, the Tribonacci lazy form below 2^44."""
+< 2^53 for the odd part o of the digit multiplier m, the 128-word stride for
+the doubling families, the Tribonacci lazy form below 2^44."""
 import os
 import sys
 
@@ -50,8 +50,8 @@ def test_smarandache_shifted_and_general_windows():
 
 def test_stride_forms_by_prime_size():
     t = bench.ops_table()
-    shift, words = 1.1875, 138 / 128
-    assert bench.window_ops(t, "proth", 1 << 30, 1 << 31) == shift
+    words = 138 / 128
+    assert bench.window_ops(t, "proth", 1 << 30, 1 << 31) == words
     assert bench.window_ops(t, "proth", 1 << 40, 1 << 41) == words
     assert bench.window_ops(t, "tribonacci", 1 << 40, 1 << 41) == 4.5
     assert bench.window_ops(t, "tribonacci", 1 << 45, 1 << 46) == 5.5

@@ -318,16 +318,18 @@ def test_small_batches_and_resume_accumulate(engine):
     assert last1 == int(o.sieve((1 << 21) - 100, 1 << 21)[-1])
 
 
-# Windows for the stride forms of the doubling and fixed-n families: below
-# 2^37 (16 indices per reduction), below 2^41 (12), below 2^45 (8), below
-# 2^48 (4), and windows straddling 2^37, 2^41 and 2^45 so one wave holds
-# primes of both ranges.
+# Windows for the stride forms of the doubling and fixed-n families, from
+# 2^36 to 2^48, including windows just below 2^37, 2^41 and 2^45 (earlier
+# kernels changed form there; every window takes the word strides now).
 STRIDE_WINDOWS = [(1 << 36) + 24690, (1 << 37) - 2000, (1 << 40) + (1 << 39), (1 << 41) - 2000,
                   (1 << 44) + (1 << 40), (1 << 45) - 2000, (1 << 47) + (1 << 30)]
-# Offsets of the planted hit from n0: the check before the loop, every
-# position of a 12- and a 16-index block, and the single-step tails (N = 600:
-# 16-index blocks cover offsets 1..592, 12-index blocks 1..588).
-STRIDE_OFFSETS = [0, 1, 2, 5, 11, 12, 13, 15, 16, 17, 63, 64, 65, 300, 575, 576, 577, 588, 589, 592, 593, 599]
+# Offsets of the planted hit from n0: the check before the loop, the first,
+# last and middle positions of the 128-index blocks of the doubling word
+# stride (offsets 1..512 with N = 600), of its 64- and 16-index tail blocks
+# (513..576, 577..592), of the 64- and 16-index blocks of the fixed-n stride,
+# and the single-step tails.
+STRIDE_OFFSETS = [0, 1, 2, 5, 11, 12, 13, 15, 16, 17, 63, 64, 65, 127, 128, 129, 255, 256, 257, 300, 383, 384,
+                  511, 512, 513, 575, 576, 577, 588, 589, 591, 592, 593, 599]
 
 
 @pytest.mark.parametrize("seq,flags", [(o.PROTH, None), (o.RIESEL, None), (o.PROTH_FIXED_N, None),
