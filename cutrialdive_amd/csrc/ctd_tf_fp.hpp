@@ -39,7 +39,7 @@
 //   fixed n  x = x + c        1.08 (6) when stepped; by default the closed
 //                             form of ctd_tf_fast.hpp solves once per prime
 //   fibonacci / lucas         1.30 / 1.27 theta stride form, 5 pair form (6)
-//   tribonacci                4.5 (p < 2^44), 5.5 (11.25)
+//   tribonacci                4.36 (p < 2^43), 4.5 (p < 2^44), 5.5 (11.25)
 //   smarandache r m + n + 1   7   (12.25), for p > 8 n1; 5 when p (0.75 m + 1) < 2^53
 //                             (short step) or (p / 2 + n1) o < 2^53 (shifted step,
 //                             o the odd part of m)
@@ -262,8 +262,44 @@ constexpr int THETA_STRIDE = FAM == FAM_LUCAS ? 128 : 112;
 constexpr uint64_t THETA_MIN_STEPS = 1024;
 // target words of the doubling families' word stride
 constexpr int DOUBLING_WORDS = 128;
-// primes below which the Tribonacci lazy form keeps 8 unreduced sums exact
+// primes below which the Tribonacci lazy form keeps 8 (11) unreduced terms exact
 constexpr uint64_t TRIB_LAZY_PMAX = 1ull << 44;
+constexpr uint64_t TRIB_LAZY11_PMAX = 1ull << 43;
+
+// One block of the Tribonacci lazy form: L indices from the reduced state
+// (a, b, c) = (T(n), T(n+1), T(n+2)), |.| <= 0.75 p.  The terms
+// x[j] = T(n+3+j) are formed unreduced: T(n+3) = a + b + c, then
+// T(k+4) = 2 T(k+3) - T(k) (subtract T(k+3) = T(k+2) + T(k+1) + T(k) from
+// T(k+4) = T(k+3) + T(k+2) + T(k+1)), one FMA per term, exact while the term
+// fits in 53 bits.  In terms of the state, T(n+3+j) = A a + B b + C c with
+// A + B + C = 3, 5, 9, 17, 31, 57, 105, 193, 355, 653, 1201 for j = 0..10, so
+// |x[7]| <= 145 p < 2^53 for p < 2^44 (L = 8) and |x[10]| <= 901 p < 2^53
+// for p < 2^43 (L = 11).  Each x[j] is tested without reducing it: with q the
+// biased rounded quotient, w = fma(q, -p, M p) = -round(x/p) p exactly
+// (|w| < 2^53), and x = 0 mod p <=> x == -w.  Only the new state
+// x[L-3..L-1] is reduced (x + w, |.| <= p/2 + tiny).  T(n+1) and T(n+2) are
+// reduced already and compare with 0.  4 + 4/L instructions per index.
+template <int L>
+__device__ __forceinline__ void trib_lazy_block(const Fmod &F, double pM, double &a, double &b, double &c,
+                                                uint64_t (&mk)[L])
+{
+    double x[L], w[L];
+    x[0] = a + (b + c);
+    x[1] = fma(2.0, x[0], -a);
+    x[2] = fma(2.0, x[1], -b);
+    x[3] = fma(2.0, x[2], -c);
+    _Pragma("unroll")
+    for (int j = 4; j < L; ++j) x[j] = fma(2.0, x[j - 1], -x[j - 4]);
+    mk[0] = __ballot(b == 0.0);
+    mk[1] = __ballot(c == 0.0);
+    _Pragma("unroll")
+    for (int j = 0; j < L; ++j) w[j] = fma(fma(x[j], F.invp, Fmod::MAGIC), -F.p, pM);
+    _Pragma("unroll")
+    for (int j = 0; j < L - 2; ++j) mk[2 + j] = __ballot(x[j] == -w[j]);
+    a = x[L - 3] + w[L - 3];
+    b = x[L - 2] + w[L - 2];
+    c = x[L - 1] + w[L - 1];
+}
 
 // v in a vector register, opaque to the compiler: a rare replay path that
 // starts from it cannot reuse the values (wave masks) its loop computed from
@@ -919,42 +955,17 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
         trib_triple_fp(F, ns, a, b, c);
         record_hits(J, idx, p, a == 0.0);
         if (__ballot(p >= TRIB_LAZY_PMAX) == 0) {
-            // Lazy form, 8 indices per block (36 instructions): from the
-            // reduced state (T(n), T(n+1), T(n+2)), |.| <= 0.75 p, the terms
-            // x[j] = T(n+3+j) are formed unreduced: T(n+3) = a + b + c, then
-            // T(k+4) = 2 T(k+3) - T(k) (subtract T(k+3) = T(k+2) + T(k+1) + T(k)
-            // from T(k+4) = T(k+3) + T(k+2) + T(k+1)), one exact FMA per term;
-            // |x[j]| <= 2 |x[j-1]| + |x[j-4]| gives |x[7]| <= 456.75 p < 2^53
-            // for p < 2^44, so every term is exact.  Each x[j] is tested
-            // without reducing it: with q the biased rounded quotient,
-            // w = fma(q, -p, M p) = -round(x/p) p exactly (|w| < 2^53), and
-            // x = 0 mod p <=> x == -w.  Only the new state x[5..7] is reduced
-            // (x + w, |.| <= p/2 + tiny).  T(n+1) and T(n+2) are reduced already
-            // and compare with 0.
+            // lazy form (trib_lazy_block): 11 indices per block below 2^43, 8 below 2^44
             const double pM = F.p * Fmod::MAGIC;
-            CTD_UNROLLED_LOOP_MASKSB(8,
-                {
-                    double x[8];
-                    double w[8];
-                    x[0] = a + (b + c);
-                    x[1] = fma(2.0, x[0], -a);
-                    x[2] = fma(2.0, x[1], -b);
-                    x[3] = fma(2.0, x[2], -c);
-                    x[4] = fma(2.0, x[3], -x[0]);
-                    x[5] = fma(2.0, x[4], -x[1]);
-                    x[6] = fma(2.0, x[5], -x[2]);
-                    x[7] = fma(2.0, x[6], -x[3]);
-                    mk[0] = __ballot(b == 0.0);
-                    mk[1] = __ballot(c == 0.0);
-                    _Pragma("unroll")
-                    for (int j = 0; j < 8; ++j) w[j] = fma(fma(x[j], F.invp, Fmod::MAGIC), -F.p, pM);
-                    _Pragma("unroll")
-                    for (int j = 0; j < 6; ++j) mk[2 + j] = __ballot(x[j] == -w[j]);
-                    a = x[5] + w[5];
-                    b = x[6] + w[6];
-                    c = x[7] + w[7];
-                },
-                { const double t = F.red(a + b + c); a = b; b = c; c = t; }, __ballot(a == 0.0))
+            if (__ballot(p >= TRIB_LAZY11_PMAX) == 0) {
+                CTD_UNROLLED_LOOP_MASKSB(11, trib_lazy_block<11>(F, pM, a, b, c, mk),
+                                         { const double t = F.red(a + b + c); a = b; b = c; c = t; },
+                                         __ballot(a == 0.0))
+            } else {
+                CTD_UNROLLED_LOOP_MASKSB(8, trib_lazy_block<8>(F, pM, a, b, c, mk),
+                                         { const double t = F.red(a + b + c); a = b; b = c; c = t; },
+                                         __ballot(a == 0.0))
+            }
             return F.canon(a, p);
         }
         // two steps share one sum: with s = T(n+1) + T(n+2),

@@ -3,7 +3,8 @@
 The model must pick, for each prime window, the loop form the kernels pick:
 the Smarandache short or shifted step (5 instructions) where (p / 2 + n1) o
 < 2^53 for the odd part o of the digit multiplier m, the 128-word stride for
-the doubling families, the Tribonacci lazy form below 2^44."""
+the doubling families, the Tribonacci lazy form's 11-index blocks below 2^43
+and 8-index blocks below 2^44."""
 import os
 import sys
 
@@ -53,7 +54,8 @@ def test_stride_forms_by_prime_size():
     words = 138 / 128
     assert bench.window_ops(t, "proth", 1 << 30, 1 << 31) == words
     assert bench.window_ops(t, "proth", 1 << 40, 1 << 41) == words
-    assert bench.window_ops(t, "tribonacci", 1 << 40, 1 << 41) == 4.5
+    assert bench.window_ops(t, "tribonacci", 1 << 40, 1 << 41) == 48 / 11
+    assert bench.window_ops(t, "tribonacci", 1 << 43, 1 << 44) == 4.5
     assert bench.window_ops(t, "tribonacci", 1 << 45, 1 << 46) == 5.5
     assert bench.window_ops(t, "fibonacci", 1 << 40, 1 << 41, 5000) == t["loop"]["fibonacci"]["fp64_theta"]
 

@@ -386,6 +386,44 @@ def test_theta_form_fibonacci_lucas_large_primes(engine, m, q):
     assert_same(g, gc, o.LUCAS, 0, n0, n0 + 2000, f0, f1)
 
 
+def _trib_factor_pairs():
+    """One pair (n, q), q | T(n), per bit length of q from 32 to 48, and every
+    pair with q in [2^42, 2^44), next to the limits of the lazy form's 11-
+    and 8-index blocks."""
+    import json
+    import os
+
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "trib_large_factors.json")
+    with open(path) as f:
+        pairs = [tuple(x) for x in json.load(f)["pairs"]]
+    first = {}
+    for n, q in pairs:
+        first.setdefault(q.bit_length(), (n, q))
+    return sorted(set(first.values()) | {(n, q) for n, q in pairs if 43 <= q.bit_length() <= 44},
+                  key=lambda z: z[1])
+
+
+@pytest.mark.parametrize("n,q", _trib_factor_pairs())
+def test_tribonacci_planted_hits(engine, n, q):
+    """q | T(n) planted at every position of a 30-index job: the check before
+    the loop, each position of the lazy form's 11-index (q < 2^43) or 8-index
+    (q < 2^44) blocks, or of the pair form's blocks (q >= 2^44), and the
+    single-step tail."""
+    T = [0, 0, 1]
+    while len(T) <= n:
+        T.append(T[-1] + T[-2] + T[-3])
+    assert T[n] % q == 0
+    f0, f1 = q - 600, q + 600
+    N = 30
+    lo = n - (N - 1)
+    want = o.trial_factor(o.TRIBONACCI, 0, lo, n + N, f0, f1)
+    for off in range(N):
+        n0 = n - off
+        g, _, _ = gpu_lists(engine, o.TRIBONACCI, 0, n0, n0 + N, f0, f1)
+        assert q in g[off], (q, off)
+        assert g == want[n0 - lo : n0 - lo + N], (q, off)
+
+
 @pytest.mark.parametrize("seq", [o.FIBONACCI, o.LUCAS])
 @pytest.mark.parametrize("n0,n1,f0,f1,chunk", [
     (1, 5000, 0, 1 << 18, 0),               # small primes (hits everywhere), 5 in the window

@@ -217,11 +217,12 @@ def test_smarandache_exact_product_limit(engine, base):
                          nat.CTD_RUN_SM_NO_SHIFT | nat.CTD_RUN_SM_NO_SHORT, nat.CTD_RUN_SM_FULL_STEP))
 
 
-def test_tribonacci_lazy_form_limit(engine):
-    # The 8-index lazy Tribonacci form runs on waves whose primes are all
-    # below 2^44; a wave of primes just below, one just above, and waves that
-    # interleave both (which take the pair form).
-    lim = 1 << 44
+@pytest.mark.parametrize("lim", [1 << 43, 1 << 44])
+def test_tribonacci_lazy_form_limit(engine, lim):
+    # The lazy Tribonacci form takes 11-index blocks on waves whose primes are
+    # all below 2^43 and 8-index blocks below 2^44 (the pair form above); a
+    # wave of primes just below the limit, one just above, and waves that
+    # interleave both (which take the next form).
     below, above = [], []
     p = lim
     while len(below) < 96:
@@ -233,7 +234,7 @@ def test_tribonacci_lazy_form_limit(engine):
         above.append(p)
     mixed = [x for pair in zip(below[64:], above[64:]) for x in pair]
     primes = below[:64] + above[:64] + mixed
-    for n0, n1 in ((5, 2100), (1, 9), (3, 11), (1000, 1017)):
+    for n0, n1 in ((5, 2100), (1, 9), (3, 11), (1000, 1017), (7, 18), (2, 27)):
         want = o.residues_last(o.TRIBONACCI, 0, n0, n1 - n0, primes)
         got = engine.residues(o.TRIBONACCI, 0, n0, n1, primes, 0)
-        assert np.array_equal(got, want), (n0, n1)
+        assert np.array_equal(got, want), (lim, n0, n1)
