@@ -39,7 +39,7 @@
 //   fixed n  x = x + c        1.08 (6) when stepped; by default the closed
 //                             form of ctd_tf_fast.hpp solves once per prime
 //   fibonacci / lucas         1.30 / 1.27 theta stride form, 5 pair form (6)
-//   tribonacci                4.36 (p < 2^43), 4.5 (p < 2^44), 5.5 (11.25)
+//   tribonacci                3.64 (p < 2^43), 3.875 (p < 2^44), 5.5 (11.25)
 //   smarandache r m + n + 1   7   (12.25), for p > 8 n1; 5 when p (0.75 m + 1) < 2^53
 //                             (short step) or (p / 2 + n1) o < 2^53 (shifted step,
 //                             o the odd part of m)
@@ -274,16 +274,33 @@ constexpr uint64_t TRIB_LAZY11_PMAX = 1ull << 43;
 // fits in 53 bits.  In terms of the state, T(n+3+j) = A a + B b + C c with
 // A + B + C = 3, 5, 9, 17, 31, 57, 105, 193, 355, 653, 1201 for j = 0..10, so
 // |x[7]| <= 145 p < 2^53 for p < 2^44 (L = 8) and |x[10]| <= 901 p < 2^53
-// for p < 2^43 (L = 11).  Each x[j] is tested without reducing it: with q the
-// biased rounded quotient, w = fma(q, -p, M p) = -round(x/p) p exactly
-// (|w| < 2^53), and x = 0 mod p <=> x == -w.  Only the new state
-// x[L-3..L-1] is reduced (x + w, |.| <= p/2 + tiny).  T(n+1) and T(n+2) are
-// reduced already and compare with 0.  4 + 4/L instructions per index.
+// for p < 2^43 (L = 11).
+//
+// The terms that do not become state, x[0..L-4], are tested by a filter of
+// one FMA and one 32-bit compare: y = x / p + 1.5 2^20 rounded lies in
+// (2^20, 2^21) (|x / p| <= 901), where a double's low word holds the
+// fraction of y to 2^-32.  For x = i p, x (1/p rounded) = i (1 + d) with
+// |d| <= 2^-53, within 901 2^-53 < 2^-33 of the integer i, so y rounds to
+// i + 1.5 2^20 and its low word is 0: the filter never misses a hit.  It
+// also passes the rare x within about p 2^-33 of a multiple of p (a 2^-32
+// chance for a term that is not a multiple; none for p < 2^32), so a block
+// whose filter masks are not empty tests those terms again exactly.
+// The last three terms are reduced for the next state: with q the biased
+// rounded quotient, w = fma(q, -p, M p) = -round(x/p) p exactly
+// (|w| < 2^53), x = 0 mod p <=> x == -w, and x + w is the reduced value
+// (|.| <= p/2 + tiny).  T(n+1) and T(n+2) are reduced already and compare
+// with 0.  3 + 7/L instructions per index.
+constexpr double TRIB_FILTER_BIAS = 1572864.0; // 1.5 * 2^20
+
+__device__ __forceinline__ double trib_neg_round(const Fmod &F, double pM, double x)
+{
+    return fma(fma(x, F.invp, Fmod::MAGIC), -F.p, pM);
+}
+
 template <int L>
 __device__ __forceinline__ void trib_lazy_block(const Fmod &F, double pM, double &a, double &b, double &c,
-                                                uint64_t (&mk)[L])
+                                                double (&x)[L], uint64_t (&mk)[L])
 {
-    double x[L], w[L];
     x[0] = a + (b + c);
     x[1] = fma(2.0, x[0], -a);
     x[2] = fma(2.0, x[1], -b);
@@ -293,12 +310,55 @@ __device__ __forceinline__ void trib_lazy_block(const Fmod &F, double pM, double
     mk[0] = __ballot(b == 0.0);
     mk[1] = __ballot(c == 0.0);
     _Pragma("unroll")
-    for (int j = 0; j < L; ++j) w[j] = fma(fma(x[j], F.invp, Fmod::MAGIC), -F.p, pM);
-    _Pragma("unroll")
-    for (int j = 0; j < L - 2; ++j) mk[2 + j] = __ballot(x[j] == -w[j]);
-    a = x[L - 3] + w[L - 3];
-    b = x[L - 2] + w[L - 2];
-    c = x[L - 1] + w[L - 1];
+    for (int j = 0; j < L - 3; ++j) {
+        const double y = fma(x[j], F.invp, TRIB_FILTER_BIAS);
+        mk[2 + j] = __ballot((uint32_t)__double_as_longlong(y) == 0u);
+    }
+    const double w0 = trib_neg_round(F, pM, x[L - 3]);
+    const double w1 = trib_neg_round(F, pM, x[L - 2]);
+    const double w2 = trib_neg_round(F, pM, x[L - 1]);
+    mk[L - 1] = __ballot(x[L - 3] == -w0);
+    a = x[L - 3] + w0;
+    b = x[L - 2] + w1;
+    c = x[L - 1] + w2;
+}
+
+// The Tribonacci lazy-form loop over `steps` indices after idx (the index of
+// the state a = T(n)); blocks of L, then single reduced steps.
+template <int L>
+__device__ __forceinline__ void trib_lazy_loop(const JobArgs &J, const Fmod &F, uint64_t p, double &a, double &b,
+                                               double &c, uint64_t &idx, uint64_t steps)
+{
+    const double pM = F.p * Fmod::MAGIC;
+    const uint32_t steps32 = (uint32_t)steps;
+    uint32_t k = 0;
+    for (; k + L <= steps32; k += L) {
+        double x[L];
+        uint64_t mk[L];
+        trib_lazy_block<L>(F, pM, a, b, c, x, mk);
+        uint64_t any = 0;
+        _Pragma("unroll")
+        for (int j = 0; j < L; ++j) any |= mk[j];
+        if (any) {
+            // the filtered terms again, exactly
+            _Pragma("unroll")
+            for (int j = 0; j < L - 3; ++j)
+                if (mk[2 + j]) mk[2 + j] = __ballot(x[j] == -trib_neg_round(F, pM, x[j]));
+            const uint32_t lane = __lane_id();
+            const uint64_t at = hit_base(idx, k);
+            for (int j = 0; j < L; ++j)
+                if (mk[j]) record_hits_mask(J, at + 1 + j, p, (mk[j] >> lane) & 1, mk[j]);
+        }
+    }
+    idx += k;
+    for (; k < steps32; ++k) {
+        const double t = F.red(a + b + c);
+        a = b;
+        b = c;
+        c = t;
+        ++idx;
+        record_hits(J, idx, p, a == 0.0);
+    }
 }
 
 // v in a vector register, opaque to the compiler: a rare replay path that
@@ -956,16 +1016,8 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
         record_hits(J, idx, p, a == 0.0);
         if (__ballot(p >= TRIB_LAZY_PMAX) == 0) {
             // lazy form (trib_lazy_block): 11 indices per block below 2^43, 8 below 2^44
-            const double pM = F.p * Fmod::MAGIC;
-            if (__ballot(p >= TRIB_LAZY11_PMAX) == 0) {
-                CTD_UNROLLED_LOOP_MASKSB(11, trib_lazy_block<11>(F, pM, a, b, c, mk),
-                                         { const double t = F.red(a + b + c); a = b; b = c; c = t; },
-                                         __ballot(a == 0.0))
-            } else {
-                CTD_UNROLLED_LOOP_MASKSB(8, trib_lazy_block<8>(F, pM, a, b, c, mk),
-                                         { const double t = F.red(a + b + c); a = b; b = c; c = t; },
-                                         __ballot(a == 0.0))
-            }
+            if (__ballot(p >= TRIB_LAZY11_PMAX) == 0) trib_lazy_loop<11>(J, F, p, a, b, c, idx, steps);
+            else trib_lazy_loop<8>(J, F, p, a, b, c, idx, steps);
             return F.canon(a, p);
         }
         // two steps share one sum: with s = T(n+1) + T(n+2),

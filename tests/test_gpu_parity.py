@@ -424,6 +424,37 @@ def test_tribonacci_planted_hits(engine, n, q):
         assert g == want[n0 - lo : n0 - lo + N], (q, off)
 
 
+def _trib_near_triples():
+    import json
+    import os
+
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "trib_near_residues.json")
+    with open(path) as f:
+        return [tuple(x) for x in json.load(f)["triples"]]
+
+
+@pytest.mark.parametrize("n,r,q", _trib_near_triples())
+def test_tribonacci_near_misses(engine, n, r, q):
+    """T(n) = r (mod q) for a small r != 0 at q > |r| 2^34: wherever the lazy
+    form leaves T(n) unreduced, its one-FMA filter passes the term, and the
+    exact re-test must reject it.  T(n) is placed at every position of a
+    30-index job (the blocks' filtered, exact and state positions, and the
+    tail)."""
+    T = [0, 0, 1]
+    while len(T) <= n:
+        T.append(T[-1] + T[-2] + T[-3])
+    assert (T[n] - r) % q == 0 and T[n] % q
+    f0, f1 = q - 600, q + 600
+    N = 30
+    lo = n - (N - 1)
+    want = o.trial_factor(o.TRIBONACCI, 0, lo, n + N, f0, f1)
+    for off in range(N):
+        n0 = n - off
+        g, _, _ = gpu_lists(engine, o.TRIBONACCI, 0, n0, n0 + N, f0, f1)
+        assert q not in g[off], (q, off)
+        assert g == want[n0 - lo : n0 - lo + N], (q, off)
+
+
 @pytest.mark.parametrize("seq", [o.FIBONACCI, o.LUCAS])
 @pytest.mark.parametrize("n0,n1,f0,f1,chunk", [
     (1, 5000, 0, 1 << 18, 0),               # small primes (hits everywhere), 5 in the window
