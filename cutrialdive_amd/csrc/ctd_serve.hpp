@@ -12,7 +12,8 @@
 //           staging buffer; an idle server always admits the next job.  Then:
 //           allocate the device buffers, copy the host buffers in, run
 //           the mod-2 kernel, and cut the prime window into tiles of about
-//           2^35 VALU lane-instructions (at most tile_length integers);
+//           2^35 VALU lane-instructions, into at most about 32 tiles (at
+//           most tile_length integers each);
 //   issue   each stream keeps up to SERVE_STREAM_WORK of estimated work in
 //           flight (at least one tile, at most SERVE_DEPTH): enough that it
 //           never runs dry between scheduler passes, little enough that a
@@ -39,6 +40,12 @@
 constexpr uint64_t SERVE_TILE_DEFAULT = 1ull << 26;
 // work = estimated updates x the family's VALU instructions per update
 constexpr double SERVE_TILE_WORK = 34359738368.0; // 2^35 lane-instructions: ~0.9 ms of the whole device
+// a job's window is cut into at most about this many tiles: a job with much
+// more work than SERVE_TILE_WORK per tile (a 2^30-integer window with many
+// indices) gets longer tiles instead of more of them, since a tile with few
+// primes splits its index range into chunks that each pay the per-prime
+// set-up again
+constexpr double SERVE_JOB_TILES = 32.0;
 constexpr uint64_t SERVE_BUDGET_DEFAULT = 8ull << 30;
 constexpr uint32_t SERVE_QUEUE_DEFAULT = 65536;
 constexpr uint32_t SERVE_DEPTH = 8; // at most this many tiles in flight per stream
@@ -357,11 +364,13 @@ bool admit(ctd_hip_server_impl *S, ServeJob *J)
     }
     J->ready = pool_event(S);
     HIPCHK(hipEventRecord(J->ready, S->copy_st));
-    // tiles of about SERVE_TILE_WORK (at most tile_length integers, at least
-    // one segment): a job with many indices or a costly family is cut finer,
-    // so its tiles spread over every stream instead of holding one
+    // tiles of about SERVE_TILE_WORK, or 1/SERVE_JOB_TILES of the window when
+    // that is longer (at most tile_length integers, at least one segment): a
+    // job with many indices or a costly family is cut finer, so its tiles
+    // spread over every stream instead of holding one
     const double primes_per_int = 1.0 / std::log((double)std::max<uint64_t>(q.f1, 16));
-    const double want = SERVE_TILE_WORK / ((double)N * J->plan.ops * primes_per_int); // integers
+    const double want = std::max(SERVE_TILE_WORK / ((double)N * J->plan.ops * primes_per_int), // integers
+                                 (double)(J->plan.f1 - J->plan.f0) / SERVE_JOB_TILES);
     const uint64_t segs = std::max<uint64_t>(1, (uint64_t)(want / (double)SEG_SPAN));
     const uint64_t tl = std::min<uint64_t>(S->opts.tile_length, segs * SEG_SPAN);
     if (J->plan.f0 < J->plan.f1) {
