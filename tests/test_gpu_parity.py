@@ -251,6 +251,21 @@ def test_cfg1_reduced(engine):
     assert g == r
 
 
+def test_cfg1_full_size(engine, mersenne_lists):
+    # config 1 at its full size, Mersenne n in [2, 20000) against every prime
+    # below 2^30 (1.09e12 updates), checked index by index against the lists
+    # the multiplicative orders of 2 give (p | M_n <=> ord_p(2) | n, found
+    # without stepping any index, and pinned against the oracle on the CPU)
+    print("[cfg1] mersenne n in [2, 20000), p < 2^30", flush=True)
+    want = mersenne_lists(30, 20000)
+    mx = 256
+    assert max(len(x) for x in want) <= mx
+    g, gc, last = gpu_lists(engine, o.MERSENNE, 0, 2, 20000, 0, 1 << 30, mx=mx)
+    assert np.array_equal(gc, np.array([len(x) for x in want], dtype=gc.dtype))
+    assert g == want
+    assert last == (1 << 30) - 35  # the largest prime below 2^30
+
+
 def test_cfg2_subwindow(engine):
     # Sm10 n in [10, 1000) over [2^32, 2^32 + 2^28) (config 2 sub-window,
     # 12.4 M primes); it holds zero residues, so the lists are not trivially empty

@@ -153,6 +153,15 @@ def test_trial_factor_near_2_64_planted_hit():
     assert got == want and got[20] == [p]
 
 
+def test_mersenne_orders_tool_equals_the_oracle(mersenne_lists):
+    # the order-based Mersenne lists (the full-size config-1 check of the GPU
+    # tests) against the oracle's trial factoring on a smaller case
+    want = o.trial_factor(o.MERSENNE, 0, 2, 2000, 0, 1 << 20)
+    assert mersenne_lists(20, 2000) == want
+    assert mersenne_lists(20, 2000, n0=3)[0] == [7]  # M_3 = 7 lists itself
+    assert mersenne_lists(20, 2000)[11 - 2] == [23, 89]
+
+
 def test_prime_count_2_30_and_readme_primes():
     assert o.prime_count(0, 1 << 30) == KATS["prime_counts"]["30"]
     assert int(o.sieve((1 << 37) - 200, 1 << 37)[-1]) == KATS["readme_largest_prime_below_2_37"]
