@@ -292,8 +292,8 @@ def batched_vs_plain(torch, eng, nat, cjobs, mx, flags=0, tile_length=0, streams
     HBM-resident buffers, and compares the two sets of buffers."""
     bufs = bufs or DeviceBuffers(torch, cjobs, mx)
     out = {"jobs": len(cjobs)}
-    upd = primes = graphs = tiles = 0
-    bms = build = 0.0
+    upd = primes = graphs = tiles = tuned = 0
+    bms = build = tune_ms = 0.0
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(steps):
@@ -306,12 +306,14 @@ def batched_vs_plain(torch, eng, nat, cjobs, mx, flags=0, tile_length=0, streams
         build += st.build_ms
         graphs += st.graphs
         tiles += st.tiles
+        tuned += st.tuned
+        tune_ms += st.autotune_ms
         print(f"# {label} batched step {k}: {st.tiles} tiles, {st.graphs} graph(s), graph {st.total_ms:.0f} ms, "
               f"build {st.build_ms:.0f} ms", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     out.update(value=upd / dt, seconds=dt, updates=upd, primes=primes, graph_ms=bms, build_ms=build, graphs=graphs,
-               tiles=tiles)
+               tiles=tiles, shapes_tuned=tuned, autotune_ms=tune_ms)
     if not plain:
         return out, bufs, None
     bc, bf = bufs.host()
@@ -516,13 +518,16 @@ def run_mixed(args, torch, ctd, nat):
     if args.short_jobs and not args.no_plain:
         sj = workloads.mixed_jobs(args.short_jobs, window_bits=24)
         sc = [nat.CtdJob(j.seq_id, j.max_factors, j.param, j.n0, j.n1, j.f0, j.f1) for j in sj]
-        eng.run_batch(sc[:64])  # launch shapes of the short windows' index classes
+        # warm-up, as for the main batch: the launch shapes of every (family,
+        # form, index class) of these jobs are tuned before the timed step (on
+        # the first 2^20 integers of each window)
+        eng.run_batch([nat.CtdJob(j.seq_id, j.max_factors, j.param, j.n0, j.n1, j.f0, j.f0 + (1 << 20)) for j in sj])
         sb, sbufs, (sp, _) = batched_vs_plain(torch, eng, nat, sc, mx, flags, tile, args.batch_streams, 1,
                                               label="2^24-window mix")
         del sbufs
         torch.cuda.empty_cache()
         short = {"workload": f"{len(sc)} jobs of the mixed-job distribution with 2^24-wide prime windows",
-                 "batched": {k: sb[k] for k in ("value", "seconds", "updates", "tiles", "graphs")},
+                 "batched": {k: sb[k] for k in ("value", "seconds", "updates", "tiles", "graphs", "shapes_tuned")},
                  "plain": {k: sp[k] for k in ("value", "seconds", "updates", "factors_identical_to_batched")},
                  "batched_vs_plain": sb["batched_vs_plain"]}
 
@@ -603,6 +608,7 @@ def run_mixed(args, torch, ctd, nat):
             "graph_launches_per_step": bat["graphs"] // max(args.steps, 1),
             "graph_build_ms_per_step": bat["build_ms"] / max(args.steps, 1),
             "autotune": not args.no_autotune,
+            "shapes_tuned_in_timed_steps": bat["shapes_tuned"],
             "shapes": {k: {"block": b, "primes_per_lane": w, "chunk_split": s} for k, (b, w, s) in sorted(shapes.items())},
         },
         "plain": result_plain,
