@@ -748,12 +748,19 @@ def run_serve(args, torch, ctd, nat):
     def pct(x):
         return {"p50_ms": float(np.percentile(x, 50)), "p99_ms": float(np.percentile(x, 99))}
 
+    # idle streams created before the served streams: HIP places a process's
+    # streams on GPU_MAX_HW_QUEUES hardware queues, and how many exist before
+    # the server decides which of its streams share one
+    extra = [torch.cuda.Stream() for _ in range(args.serve_extra_streams)]
+    # offered rates: fractions of the measured capacity, or given in jobs/s
+    rates = [float(x) for x in args.serve_rates.split(",")] if args.serve_rates else [L * capacity for L in loads]
     runs = []
-    for L in loads:
-        arr = workloads.poisson_arrivals(len(cjobs), L * capacity, seed=workloads.SEED + int(L * 1000))
+    for rate in rates:
+        L = rate / capacity
+        arr = workloads.poisson_arrivals(len(cjobs), rate, seed=workloads.SEED + int(round(L * 1000)))
         lat, dig, el, st = serve(cjobs, arr)
         lat_np = np.array(lat)
-        runs.append({"load": L, "offered_jobs_per_s": L * capacity, "jobs": len(cjobs), "seconds": el,
+        runs.append({"load": L, "offered_jobs_per_s": rate, "jobs": len(cjobs), "seconds": el,
                      "p50_ms": float(np.percentile(lat_np, 50)), "p99_ms": float(np.percentile(lat_np, 99)),
                      "mean_ms": float(lat_np.mean()),
                      "by_work": {"largest_10pct": pct(lat_np[large]), "other_90pct": pct(lat_np[~large])},
@@ -764,7 +771,8 @@ def run_serve(args, torch, ctd, nat):
                      "device_bytes_peak": st.device_bytes_peak, "pinned_bytes_peak": st.pinned_bytes_peak,
                      "digests": dig})
         bw = runs[-1]["by_work"]
-        print(f"# load {L}: {el:.1f} s, p50 {runs[-1]['p50_ms']:.1f} ms, p99 {runs[-1]['p99_ms']:.1f} ms "
+        print(f"# load {L:.2f} ({rate:.0f} jobs/s): {el:.1f} s, p50 {runs[-1]['p50_ms']:.1f} ms, "
+              f"p99 {runs[-1]['p99_ms']:.1f} ms "
               f"(largest 10 %: {bw['largest_10pct']['p50_ms']:.1f} / {bw['largest_10pct']['p99_ms']:.1f}, "
               f"others: {bw['other_90pct']['p50_ms']:.1f} / {bw['other_90pct']['p99_ms']:.1f}), "
               f"{st.updates / el:.3e} updates/s, active max {st.active_max}, scheduler "
@@ -810,7 +818,7 @@ def run_serve(args, torch, ctd, nat):
         "value": top["updates_per_s"],
         "unit": "updates/s",
         "n_gpus": 1,
-        "steps": len(loads),
+        "steps": len(runs),
         "warmup": 0,
         "ms_per_step": top["seconds"] * 1e3,
         "higher_is_better": True,
@@ -822,8 +830,10 @@ def run_serve(args, torch, ctd, nat):
         "data": "synthetic (seeded job generator and Poisson arrivals)",
         "config": {
             "workload": f"serving: Poisson stream of {len(cjobs)} jobs of the mixed-job distribution with 2^24-wide "
-                        f"prime windows, offered load {loads} of measured capacity; value = sustained updates/s "
-                        f"at load {top['load']}",
+                        f"prime windows, offered "
+                        + (f"rates {rates} jobs/s" if args.serve_rates else f"load {loads} of measured capacity")
+                        + f"; value = sustained updates/s at {top['offered_jobs_per_s']:.0f} jobs/s",
+            "extra_streams": args.serve_extra_streams,
             "capacity_jobs_per_s": capacity,
             "capacity_updates_per_s": cap_st.updates / cap_s,
             "capacity_sample_jobs": ncap,
@@ -1011,6 +1021,9 @@ def main() -> int:
     ap.add_argument("--workload", default="mixed", choices=["mixed", "serve", "copy", "probe"] + sorted(SINGLE))
     ap.add_argument("--serve-jobs", type=int, default=10000, help="serve: jobs per offered load")
     ap.add_argument("--loads", default="0.5,0.7,0.9", help="serve: offered loads (fractions of capacity)")
+    ap.add_argument("--serve-rates", default="", help="serve: offered rates in jobs/s (in place of --loads)")
+    ap.add_argument("--serve-extra-streams", type=int, default=0,
+                    help="serve: idle streams created before the served ones (hardware-queue placement)")
     ap.add_argument("--capacity-jobs", type=int, default=2000, help="serve: jobs of the capacity measurement")
     ap.add_argument("--serve-streams", type=int, default=3,
                     help="serve and the serving block of mixed: streams (resident prime slots)")
