@@ -167,3 +167,189 @@ def test_block_masks_equal_exact_divisibility(L, limit):
             for s, t in zip(state, true[L - 3 :]):
                 assert (int(s) - t) % p == 0
                 assert abs(s) <= p / 2 + 2
+
+
+# ---------------------------------------------------------------- Smarandache steps
+#
+# run_fp_sm (ctd_tf_fp.hpp) steps r_{n+1} = r_n m + (n+1) mod p with
+# m = b^digits(n+1) < 2^21, c = n+1 < n1 <= 2^32 and p > 8 n1, p < 2^48, in
+# one of four forms chosen per segment by the bounds below.  Each emulation
+# checks that every operation the kernel relies on is exact (its rounded
+# result equals the exact rational value), that the new state is congruent
+# to the true residue, and that it stays inside the bound the next step
+# assumes.
+
+SM_SHORT_LIMIT = 9007199254740992.0 - 1073741824.0  # 2^53 - 2^30
+SM_EXACT_HALF = 0.5000001
+
+
+def exact(value, want):
+    assert Fraction(value) == want, (value, want)
+    return value
+
+
+def sm_short(v, m, c, p):
+    """Short step (p (0.75 m + 1) < 2^53 - 2^30): k1 = fma(v, m/p, M),
+    v' = fma(v, m, fma(k1, -p, M p)) + c."""
+    pf, mf = float(p), float(m)
+    mp = mf * (1.0 / pf)
+    pM = exact(pf * MAGIC, Fraction(p) * Fraction(MAGIC))
+    k1 = fma(v, mp, MAGIC)
+    w = exact(fma(k1, -pf, pM), -(Fraction(k1) - Fraction(MAGIC)) * p)
+    x = exact(fma(v, mf, w), Fraction(v) * m + Fraction(w))
+    return exact(x + c, Fraction(x) + c)
+
+
+def sm_exact_product(v, m, c, p):
+    """Exact-product step (((1/2 + 2^-30) p + n1) o < 2^53, o the odd part
+    of m): r m is exact, v' = fma(-k, p, r m) + c."""
+    pf, mf = float(p), float(m)
+    mp = mf * (1.0 / pf)
+    rm = exact(v * mf, Fraction(v) * m)
+    k = fma(v, mp, MAGIC) - MAGIC
+    x = exact(fma(-k, pf, rm), Fraction(rm) - Fraction(k) * p)
+    return exact(x + c, Fraction(x) + c)
+
+
+def sm_general(v, m, c, p):
+    """General step: k = round(v m / p), u = fma(v, m, -(k p_hi)) exact with
+    p_hi the top 32 significant bits of p, v' = fma(-k, p_lo, u + c)."""
+    pf, mf = float(p), float(m)
+    mp = mf * (1.0 / pf)
+    L = p.bit_length()
+    ph = p & ~((1 << (L - 32)) - 1) if L > 32 else p
+    pl = p - ph
+    k = fma(v, mp, MAGIC) - MAGIC
+    kph = exact(k * float(ph), Fraction(k) * ph)
+    u = exact(fma(v, mf, -kph), Fraction(v) * m - Fraction(kph))
+    uc = exact(u + c, Fraction(u) + c)
+    return exact(fma(-k, float(pl), uc), Fraction(uc) - Fraction(k) * pl)
+
+
+def sm_shifted(t, m, n, p):
+    """Shifted step on t = r - n: rm = fma(t, m, n m) exact, t' = rm - k p =
+    S(n+1) - (n+1) mod p; t' == 0 tests index n."""
+    pf, mf = float(p), float(m)
+    nm = exact(float(n) * mf, Fraction(n) * m)
+    rm = exact(fma(t, mf, nm), Fraction(t) * m + Fraction(nm))
+    k = fma(rm, 1.0 / pf, MAGIC) - MAGIC
+    return exact(fma(-k, pf, rm), Fraction(rm) - Fraction(k) * p)
+
+
+def digits(n, b):
+    d = 0
+    while n:
+        n //= b
+        d += 1
+    return d
+
+
+def sm_residue(b, n, p):
+    r = 0
+    for i in range(1, n + 1):
+        r = (r * b ** digits(i, b) + i) % p
+    return r
+
+
+def odd_part(m):
+    while m % 2 == 0:
+        m //= 2
+    return m
+
+
+def run_sm_form(form, b, n0, n1, p):
+    """Steps n0 -> n1 - 1 in one form from S(n0) mod p; every state is
+    checked against the true residue and its bound."""
+    r = sm_residue(b, n0, p)
+    v = float(r if r <= p // 2 else r - p)
+    shifted = form is sm_shifted
+    if shifted:
+        v = v - n0  # t = r - n
+    zeros = 0
+    for n in range(n0, n1 - 1):
+        m = b ** digits(n + 1, b)
+        true_next = (r * m + n + 1) % p
+        if shifted:
+            t2 = form(v, m, n, p)
+            # t2 = S(n+1) - (n+1) and it is 0 exactly when S(n) is
+            assert (int(t2) - (true_next - (n + 1))) % p == 0
+            assert (t2 == 0.0) == (r == 0)
+            assert abs(t2) <= SM_EXACT_HALF * p
+            v = t2
+        else:
+            v2 = form(v, m, n + 1, p)
+            assert (int(v2) - true_next) % p == 0
+            bound = 0.75 * p if form is not sm_exact_product else SM_EXACT_HALF * p + n1
+            assert abs(v2) <= bound
+            assert (v2 == 0.0) == (true_next == 0)
+            v = v2
+        zeros += r == 0
+        r = true_next
+    return zeros
+
+
+def segment(b, m, n0, n1, length):
+    """[lo, hi): steps n -> n+1 inside [n0, n1) that share the multiplier m
+    (the kernel picks a form per such segment), at most `length` of them."""
+    ns = [n for n in range(n0, n1 - 1) if b ** digits(n + 1, b) == m]
+    return ns[0], min(ns[-1] + 2, ns[0] + length + 1)
+
+
+def largest_prime_below(x):
+    import sympy
+
+    return int(sympy.prevprime(int(x)))
+
+
+@pytest.mark.parametrize("b", [10, 13, 3, 16, 7])
+def test_smarandache_short_step_at_its_limit(b):
+    # the largest prime the short step admits for each multiplier of [n0, n1)
+    n0, n1 = 90, 1200
+    for m in sorted({b ** digits(n, b) for n in range(n0 + 1, n1)}):
+        p = largest_prime_below(SM_SHORT_LIMIT / (0.75 * m + 1.0))
+        if p < 8 * n1 or p >= 1 << 48:
+            continue
+        assert p * (0.75 * m + 1.0) < SM_SHORT_LIMIT
+        lo, hi = segment(b, m, n0, n1, 300)
+        run_sm_form(sm_short, b, lo, hi, p)
+
+
+@pytest.mark.parametrize("b", [10, 6, 12, 4, 2])
+def test_smarandache_exact_product_and_shifted_steps_at_their_limit(b):
+    n0, n1 = 90, 1200
+    for m in sorted({b ** digits(n, b) for n in range(n0 + 1, n1)}):
+        o = odd_part(m)
+        p = largest_prime_below((SM_SHORT_LIMIT / o - n1) / SM_EXACT_HALF)
+        if p >= 1 << 48:
+            p = largest_prime_below(1 << 48)
+        assert (p * SM_EXACT_HALF + n1) * o < SM_SHORT_LIMIT
+        lo, hi = segment(b, m, n0, n1, 200)
+        run_sm_form(sm_exact_product, b, lo, hi, p)
+        run_sm_form(sm_shifted, b, max(lo, 1), hi, p)
+
+
+@pytest.mark.parametrize("b", [13, 15, 9, 10, 11])
+def test_smarandache_general_step_large_multipliers(b):
+    # the largest narrow multipliers (up to 13^5, 15^5) with primes up to 2^48
+    rng = random.Random(b)
+    for n0 in (b ** 3 - 3, b ** 4 - 3, 100003):
+        m = b ** digits(n0 + 2, b)
+        if m >= 1 << 21:
+            continue
+        n1 = n0 + 150
+        for p in (largest_prime_below(1 << 48), largest_prime_below(1 << 43),
+                  int(__import__("sympy").nextprime(rng.randrange(max(8 * n1, 1 << 38), 1 << 47)))):
+            run_sm_form(sm_general, b, n0, n1, p)
+
+
+def test_smarandache_steps_find_planted_zeros():
+    # small primes above 8 n1, where zeros are frequent: every form (within
+    # its bound) reports exactly the true zeros
+    import sympy
+
+    b, n0, n1 = 10, 20, 700
+    total = 0
+    for p in sympy.primerange(8 * n1 + 1, 8 * n1 + 400):
+        for form in (sm_short, sm_exact_product, sm_general, sm_shifted):
+            total += run_sm_form(form, b, n0, n1, p)
+    assert total > 0
