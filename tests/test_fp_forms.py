@@ -353,3 +353,56 @@ def test_smarandache_steps_find_planted_zeros():
         for form in (sm_short, sm_exact_product, sm_general, sm_shifted):
             total += run_sm_form(form, b, n0, n1, p)
     assert total > 0
+
+
+# ---------------------------------------------------------------- doubling stride arithmetic
+#
+# The doubling families' word stride (ctd_tf_fp.hpp) keeps x = k 2^n mod p
+# canonical (|x| <= (p-1)/2), so that equal residues have equal doubles and
+# their low words can be compared against precomputed targets t 2^-j.  The
+# targets come from half_canon, the state advances by mulc_canon with
+# C = 2^128 mod p.
+
+
+def quot(v, p):
+    return fma(v, 1.0 / float(p), MAGIC) - MAGIC
+
+
+def red(v, p):
+    return fma(-quot(v, p), float(p), v)
+
+
+def mulc_canon(x, C, p):
+    hi = x * C
+    lo = exact(fma(x, C, -hi), Fraction(x) * Fraction(C) - Fraction(hi))
+    t = exact(fma(-quot(hi, p), float(p), hi), Fraction(hi) - Fraction(quot(hi, p)) * p)
+    return red(exact(t + lo, Fraction(t) + Fraction(lo)), p)
+
+
+def half_canon(t, p):
+    import math
+
+    h = 0.5 * t
+    f = h - math.floor(h)
+    return fma(-f, math.copysign(float(p), t), h)
+
+
+def centered(r, p):
+    r %= p
+    return r - p if r > p // 2 else r
+
+
+@pytest.mark.parametrize("limit", [1 << 36, 1 << 44, 1 << 48])
+def test_mulc_canon_and_half_canon_are_exact_and_canonical(limit):
+    rng = random.Random(limit)
+    for p in primes_near(limit, 10, rng)[:14]:
+        h = (p - 1) // 2
+        cases = [(h, h), (-h, h), (h, -h), (1, h), (h - 1, 2), (-1, -1)]
+        cases += [(rng.randint(-h, h), rng.randint(-h, h)) for _ in range(40)]
+        for x, C in cases:
+            got = mulc_canon(float(x), float(C), p)
+            assert got == centered(x * C, p), (p, x, C)
+            assert abs(got) <= h
+        for t in [h, -h, 1, -1, 0, 2] + [rng.randint(-h, h) for _ in range(40)]:
+            got = half_canon(float(t), p)
+            assert got == centered(t * pow(2, -1, p), p), (p, t)
