@@ -685,7 +685,7 @@ def run_serve(args, torch, ctd, nat):
         """Submits jobs at the arrival times (None: all at once); returns
         (latencies ms, digests, seconds from first submit to last completion, stats)."""
         srv = ctd.Server(eng, tile_length=1 << 24, streams=args.serve_streams, fresh_buffers=True,
-                         fifo=args.serve_fifo, depth=args.serve_depth)
+                         fifo=args.serve_fifo, depth=args.serve_depth, own_queues=args.serve_own_queues)
         lat = [0.0] * len(jobs)
         dig = [None] * len(jobs)
         tickets = [None] * len(jobs)
@@ -842,6 +842,7 @@ def run_serve(args, torch, ctd, nat):
             "tile_length": 1 << 24,
             "streams": args.serve_streams,
             "policy": "fifo" if args.serve_fifo else "alternately the oldest job and the least remaining work",
+            "own_queues": args.serve_own_queues,
             "depth": args.serve_depth or 2,
         },
         "serving": runs,
@@ -1030,6 +1031,8 @@ def main() -> int:
     ap.add_argument("--serve-depth", type=int, default=0,
                     help="serve and the serving block of mixed: tiles of work each stream keeps in flight "
                          "(0 = server default, 2)")
+    ap.add_argument("--serve-own-queues", action="store_true",
+                    help="serve: the server's streams get hardware queues of their own (CTD_SERVE_OWN_QUEUES)")
     ap.add_argument("--serve-fifo", action="store_true",
                     help="serve: tiles go to the oldest job (default: alternately the oldest and the least remaining work)")
     ap.add_argument("--serve-cpu-jobs", type=int, default=12, help="serve: served jobs checked against the oracle")

@@ -37,6 +37,7 @@ CTD_BATCH_NO_FP64 = 8
 CTD_BATCH_STEPPED = 16
 CTD_SERVE_FRESH_BUFFERS = 1
 CTD_SERVE_FIFO = 2
+CTD_SERVE_OWN_QUEUES = 4
 
 
 class CtdJob(ctypes.Structure):

@@ -991,12 +991,23 @@ int ctd_hip_server_open(ctd_hip_engine *E, const ctd_server_opts *opts, ctd_hip_
         if (!E->d_err) HIPCHK(hipMalloc((void **)&E->d_err, sizeof(unsigned long long)));
         hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, E->stream, E->d_err, 1u);
         HIPCHK(hipStreamSynchronize(E->stream));
-        HIPCHK(hipStreamCreateWithFlags(&S->copy_st, hipStreamNonBlocking));
+        // CTD_SERVE_OWN_QUEUES: a stream with a CU mask (here every CU) gets
+        // a hardware queue of its own instead of sharing one of the
+        // GPU_MAX_HW_QUEUES queues with the process's other streams, where
+        // two busy streams run in turn
+        const std::vector<uint32_t> all_cus((E->cu + 31) / 32, 0xffffffffu);
+        auto make_stream = [&](hipStream_t *st) {
+            if (S->opts.flags & CTD_SERVE_OWN_QUEUES)
+                HIPCHK(hipExtStreamCreateWithCUMask(st, (uint32_t)all_cus.size(), all_cus.data()));
+            else
+                HIPCHK(hipStreamCreateWithFlags(st, hipStreamNonBlocking));
+        };
+        make_stream(&S->copy_st);
         S->slots.resize(S->opts.streams);
         S->inflight.resize(S->opts.streams);
         S->inflight_work.assign(S->opts.streams, 0.0);
         for (auto &sl : S->slots) {
-            HIPCHK(hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking));
+            make_stream(&sl.st);
             HIPCHK(hipMalloc((void **)&sl.sc, 2 * sizeof(unsigned long long)));
             ensure_slot_buffers(sl, S->opts.tile_length);
         }

@@ -344,10 +344,12 @@ class Server:
     engine handle must not be used for other calls."""
 
     def __init__(self, engine: Engine, memory_budget: int = 0, tile_length: int = 0, streams: int = 0,
-                 queue_limit: int = 0, fresh_buffers: bool = False, fifo: bool = False, depth: int = 0):
+                 queue_limit: int = 0, fresh_buffers: bool = False, fifo: bool = False, depth: int = 0,
+                 own_queues: bool = False):
         self.eng = engine
         self.lib = engine.lib
-        flags = (nat.CTD_SERVE_FRESH_BUFFERS if fresh_buffers else 0) | (nat.CTD_SERVE_FIFO if fifo else 0)
+        flags = ((nat.CTD_SERVE_FRESH_BUFFERS if fresh_buffers else 0) | (nat.CTD_SERVE_FIFO if fifo else 0)
+                 | (nat.CTD_SERVE_OWN_QUEUES if own_queues else 0))
         opts = nat.CtdServerOpts(memory_budget, tile_length, streams, queue_limit, flags, depth)
         h = ctypes.c_void_p()
         engine._check(self.lib.ctd_hip_server_open(engine.h, ctypes.byref(opts), ctypes.byref(h)))

@@ -248,6 +248,13 @@ typedef struct ctd_server_opts {
 #define CTD_SERVE_FIFO 2u          /* tiles go to the oldest admitted job (default: places
                                       alternate between the oldest job and the one with the
                                       least remaining work) */
+#define CTD_SERVE_OWN_QUEUES 4u    /* the server's streams are created with a CU mask of every
+                                      CU, which gives each a hardware queue of its own: latency
+                                      no longer depends on how the process's other streams share
+                                      the GPU_MAX_HW_QUEUES queues (default: non-blocking streams,
+                                      which HIP places on those queues with the others).  Such
+                                      streams are blocking streams: work on the null stream waits
+                                      for them and they for it */
 
 typedef struct ctd_server_stats {
     uint64_t submitted, admitted, completed, failed;

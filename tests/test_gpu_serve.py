@@ -179,6 +179,24 @@ def test_fifo_policy_gives_the_same_buffers(engine):
         assert_same(c, f, rc, rf, 64, s)
 
 
+def test_own_queues_give_the_same_buffers(engine):
+    # CTD_SERVE_OWN_QUEUES changes where the server's streams run, never a
+    # job's factors
+    jobs = small_jobs(24, seed=11, window_bits=22, nmax=3000)
+    out = {}
+    for own in (False, True):
+        print("own queues", own, flush=True)
+        srv = ctd.Server(engine, tile_length=1 << 20, streams=3, own_queues=own)
+        tickets = [srv.submit(job_of(s)) for s in jobs]
+        out[own] = [srv.wait(t)[:2] for t in tickets]
+        srv.close()
+    for s, (c0, f0), (c1, f1) in zip(jobs, out[False], out[True]):
+        assert_same(c0, f0, c1, f1, 64, s)
+    for s, (c, f) in list(zip(jobs, out[True]))[:8]:
+        rc, rf = oracle.trial_factor_raw(s.seq_id, s.param, s.n0, s.n1, s.f0, s.f1, 64)
+        assert_same(c, f, rc, rf, 64, s)
+
+
 def size_class(b):
     c = 4096
     while c < b:
