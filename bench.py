@@ -132,7 +132,7 @@ def window_ops(t, fam: str, lo: int, hi: int, n_items: int = 0, sm=None) -> floa
     the odd part of m; every bench prime is above the base): the short step
     where p (0.75 m + 1) < 2^53, else the shifted step; else the general
     step."""
-    cuts = {lo, hi} | {1 << b for b in (31, 35, 37, 41, 44, 45, 48) if lo < (1 << b) < hi}
+    cuts = {lo, hi} | {1 << b for b in (31, 35, 37, 41, 43, 44, 45, 46, 47, 48) if lo < (1 << b) < hi}
     mults = sm_multipliers(*sm) if (fam == "smarandache" and sm) else []
     lims_x = {m: int((SM_SHORT_LIMIT / odd_part(m) - sm[2]) / SM_EXACT_HALF) for m, _ in mults} if mults else {}
     cuts |= {x for x in lims_x.values() if lo < x < hi}

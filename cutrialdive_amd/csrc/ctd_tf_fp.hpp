@@ -39,7 +39,7 @@
 //   fixed n  x = x + c        1.08 (6) when stepped; by default the closed
 //                             form of ctd_tf_fast.hpp solves once per prime
 //   fibonacci / lucas         1.30 / 1.27 theta stride form, 5 pair form (6)
-//   tribonacci                3.64 (p < 2^43), 3.875 (p < 2^44), 5.5 (11.25)
+//   tribonacci                3 + 7/L: 3.64 (p < 2^43, L = 11) to 4.4 (p < 2^48, L = 5); 11.25 above
 //   smarandache r m + n + 1   7   (12.25), for p > 8 n1; 5 when p (0.75 m + 1) < 2^53
 //                             (short step) or (p / 2 + n1) o < 2^53 (shifted step,
 //                             o the odd part of m)
@@ -262,9 +262,6 @@ constexpr int THETA_STRIDE = FAM == FAM_LUCAS ? 128 : 112;
 constexpr uint64_t THETA_MIN_STEPS = 1024;
 // target words of the doubling families' word stride
 constexpr int DOUBLING_WORDS = 128;
-// primes below which the Tribonacci lazy form keeps 8 (11) unreduced terms exact
-constexpr uint64_t TRIB_LAZY_PMAX = 1ull << 44;
-constexpr uint64_t TRIB_LAZY11_PMAX = 1ull << 43;
 
 // One block of the Tribonacci lazy form: L indices from the reduced state
 // (a, b, c) = (T(n), T(n+1), T(n+2)), |.| <= 0.75 p.  The terms
@@ -273,8 +270,10 @@ constexpr uint64_t TRIB_LAZY11_PMAX = 1ull << 43;
 // T(k+4) = T(k+3) + T(k+2) + T(k+1)), one FMA per term, exact while the term
 // fits in 53 bits.  In terms of the state, T(n+3+j) = A a + B b + C c with
 // A + B + C = 3, 5, 9, 17, 31, 57, 105, 193, 355, 653, 1201 for j = 0..10, so
-// |x[7]| <= 145 p < 2^53 for p < 2^44 (L = 8) and |x[10]| <= 901 p < 2^53
-// for p < 2^43 (L = 11).
+// the last term of a block of L stays below 2^53 with L = 11 for p < 2^43
+// (|x[10]| <= 901 p), L = 10 for p < 2^44 (490 p), L = 8 for p < 2^45
+// (145 p), L = 7 for p < 2^46 (79 p), L = 6 for p < 2^47 (43 p) and L = 5
+// for p < 2^48 (24 p).
 //
 // The terms that do not become state, x[0..L-4], are tested by a filter of
 // one FMA and one 32-bit compare: y = x / p + 1.5 2^20 rounded lies in
@@ -1014,29 +1013,15 @@ __device__ uint64_t run_fp(const JobArgs &J, uint64_t p, uint64_t ns, uint64_t n
         double a, b, c;
         trib_triple_fp(F, ns, a, b, c);
         record_hits(J, idx, p, a == 0.0);
-        if (__ballot(p >= TRIB_LAZY_PMAX) == 0) {
-            // lazy form (trib_lazy_block): 11 indices per block below 2^43, 8 below 2^44
-            if (__ballot(p >= TRIB_LAZY11_PMAX) == 0) trib_lazy_loop<11>(J, F, p, a, b, c, idx, steps);
-            else trib_lazy_loop<8>(J, F, p, a, b, c, idx, steps);
-            return F.canon(a, p);
-        }
-        // two steps share one sum: with s = T(n+1) + T(n+2),
-        // T(n+3) = T(n) + s and T(n+4) = s + T(n+3) (11 instructions per 2)
-        CTD_UNROLLED_LOOP(
-            {
-                double s = b + c;
-                const double d = F.red(a + s);
-                const double e = F.red(s + d);
-                h0 = b == 0.0; h1 = c == 0.0;
-                s = d + e;
-                a = F.red(c + s);
-                b = F.red(s + a);
-                c = e;
-                h2 = d == 0.0; h3 = e == 0.0;
-                // (a, b, c) = (T(n+5), T(n+6), T(n+4)): rotate to (T(n+4), T(n+5), T(n+6))
-                const double t = c; c = b; b = a; a = t;
-            },
-            { const double t = F.red(a + b + c); a = b; b = c; c = t; }, a == 0.0)
+        // lazy form (trib_lazy_block): the longest block whose terms stay
+        // exact for every prime of the wave (0.75 S p < 2^53 with S the sum
+        // of the last term's coefficients)
+        if (__ballot(p >= (1ull << 43)) == 0) trib_lazy_loop<11>(J, F, p, a, b, c, idx, steps);
+        else if (__ballot(p >= (1ull << 44)) == 0) trib_lazy_loop<10>(J, F, p, a, b, c, idx, steps);
+        else if (__ballot(p >= (1ull << 45)) == 0) trib_lazy_loop<8>(J, F, p, a, b, c, idx, steps);
+        else if (__ballot(p >= (1ull << 46)) == 0) trib_lazy_loop<7>(J, F, p, a, b, c, idx, steps);
+        else if (__ballot(p >= (1ull << 47)) == 0) trib_lazy_loop<6>(J, F, p, a, b, c, idx, steps);
+        else trib_lazy_loop<5>(J, F, p, a, b, c, idx, steps); // p < FP_PMAX = 2^48
         return F.canon(a, p);
     } else if constexpr (FAM == FAM_SMARANDACHE) {
         const uint64_t pp[1] = {p};

@@ -54,10 +54,11 @@ def test_stride_forms_by_prime_size():
     words = 138 / 128
     assert bench.window_ops(t, "proth", 1 << 30, 1 << 31) == words
     assert bench.window_ops(t, "proth", 1 << 40, 1 << 41) == words
-    # Tribonacci lazy form: 3 L + 7 instructions per block of L indices
-    assert bench.window_ops(t, "tribonacci", 1 << 40, 1 << 41) == pytest.approx(40 / 11)
-    assert bench.window_ops(t, "tribonacci", 1 << 43, 1 << 44) == pytest.approx(31 / 8)
-    assert bench.window_ops(t, "tribonacci", 1 << 45, 1 << 46) == 5.5
+    # Tribonacci lazy form: 3 L + 7 instructions per block of L indices, L
+    # from 11 (p < 2^43) down to 5 (p < 2^48)
+    for lo, L in ((40, 11), (43, 10), (44, 8), (45, 7), (46, 6), (47, 5)):
+        assert bench.window_ops(t, "tribonacci", 1 << lo, 1 << (lo + 1)) == pytest.approx((3 * L + 7) / L)
+    assert bench.window_ops(t, "tribonacci", 1 << 48, 1 << 49) == 11.25
     assert bench.window_ops(t, "fibonacci", 1 << 40, 1 << 41, 5000) == t["loop"]["fibonacci"]["fp64_theta"]
 
 

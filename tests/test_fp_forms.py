@@ -87,21 +87,26 @@ def primes_near(limit: int, count: int, rng):
     return out
 
 
+# the block length the kernel takes below each prime limit (run_fp, FAM_TRIBONACCI)
+BLOCKS = [(1 << 43, 11), (1 << 44, 10), (1 << 45, 8), (1 << 46, 7), (1 << 47, 6), (1 << 48, 5)]
+
+
 def test_growth_bound_keeps_terms_exact():
-    # |x[j]| <= (A + B + C) 0.75 p, the sums 3, 5, 9, ..., 1201; the last
-    # term of an 11-index block stays below 2^53 for p < 2^43, of an 8-index
-    # block for p < 2^44, and |x / p| stays below 2^19, where y keeps 32
-    # fraction bits
+    # |x[j]| <= (A + B + C) 0.75 p, the sums 3, 5, 9, ..., 1201: the last
+    # term of each block length stays below 2^53 below its prime limit, the
+    # next longer block would not, and |x / p| stays below 2^19, where y
+    # keeps 32 fraction bits
     sums = [sum(t) for t in coefficients(11)]
     assert sums == [3, 5, 9, 17, 31, 57, 105, 193, 355, 653, 1201]
     assert all(min(t) >= 0 for t in coefficients(11))
-    assert 0.75 * sums[10] * 2 ** 43 < 2 ** 53
-    assert 0.75 * sums[7] * 2 ** 44 < 2 ** 53
-    assert 0.75 * sums[10] * 2 ** 44 > 2 ** 53  # an 11-index block would not be exact at 2^44
+    for limit, L in BLOCKS:
+        assert 0.75 * sums[L - 1] * limit < 2 ** 53, (limit, L)
+        if L < 11:
+            assert 0.75 * sums[L] * limit > 2 ** 53, (limit, L + 1)
     assert 0.75 * sums[10] < 2 ** 19
 
 
-@pytest.mark.parametrize("limit,L", [(1 << 43, 11), (1 << 44, 8)])
+@pytest.mark.parametrize("limit,L", BLOCKS)
 def test_filter_passes_every_multiple(limit, L):
     rng = random.Random(limit)
     bound = int(0.75 * sum(coefficients(L)[L - 1])) + 1
@@ -137,8 +142,8 @@ def test_filter_near_misses_and_small_primes():
                 assert not filter_passes(float(i * p + r), invp)
 
 
-@pytest.mark.parametrize("L,limit", [(11, 1 << 43), (8, 1 << 44)])
-def test_block_masks_equal_exact_divisibility(L, limit):
+@pytest.mark.parametrize("limit,L", BLOCKS)
+def test_block_masks_equal_exact_divisibility(limit, L):
     """Random and planted states: after the exact re-test every mask equals
     'the true integer term is divisible by p'; the filter masks contain them;
     the next state is congruent to the true terms and reduced."""

@@ -217,12 +217,12 @@ def test_smarandache_exact_product_limit(engine, base):
                          nat.CTD_RUN_SM_NO_SHIFT | nat.CTD_RUN_SM_NO_SHORT, nat.CTD_RUN_SM_FULL_STEP))
 
 
-@pytest.mark.parametrize("lim", [1 << 43, 1 << 44])
+@pytest.mark.parametrize("lim", [1 << 43, 1 << 44, 1 << 45, 1 << 46, 1 << 47, 1 << 48])
 def test_tribonacci_lazy_form_limit(engine, lim):
-    # The lazy Tribonacci form takes 11-index blocks on waves whose primes are
-    # all below 2^43 and 8-index blocks below 2^44 (the pair form above); a
-    # wave of primes just below the limit, one just above, and waves that
-    # interleave both (which take the next form).
+    # The lazy Tribonacci form takes blocks of 11 indices on waves whose
+    # primes are all below 2^43, then 10, 8, 7, 6 and 5 below 2^44 .. 2^48
+    # (the integer form above); a wave of primes just below each limit, one
+    # just above, and waves that interleave both (which take the next form).
     below, above = [], []
     p = lim
     while len(below) < 96:
