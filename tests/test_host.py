@@ -87,6 +87,14 @@ def test_python_binding_covers_the_header(built):
     assert set(header_functions()) <= set(_native.SIGNATURES)
 
 
+def test_missing_library_fails_loudly(tmp_path):
+    # the product path has no CPU fallback: without the HIP library it raises
+    from cutrialdive_amd import _native
+
+    with pytest.raises(RuntimeError, match="build the HIP engine"):
+        _native.load(str(tmp_path / "libctd_hip.so"))
+
+
 def test_abi_version_without_a_device(built):
     lib = ctypes.CDLL(LIB)
     lib.ctd_hip_abi_version.restype = ctypes.c_uint32
