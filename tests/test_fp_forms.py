@@ -411,3 +411,44 @@ def test_mulc_canon_and_half_canon_are_exact_and_canonical(limit):
         for t in [h, -h, 1, -1, 0, 2] + [rng.randint(-h, h) for _ in range(40)]:
             got = half_canon(float(t), p)
             assert got == centered(t * pow(2, -1, p), p), (p, t)
+
+
+# ---------------------------------------------------------------- Fmod (ctd_tf_fp.hpp)
+#
+# Fmod::mul(a, b) = fma(-round(hi/p), p, hi) + lo with hi = a b rounded and
+# lo = fma(a, b, -hi) its exact error (Dekker): every operation is exact and
+# the result is congruent to a b; Fmod::red(v) = fma(-round(v/p), p, v).  The
+# set-up code (powers, doubling formulas, jump-ahead) chains them on values
+# of size about p, and its results feed the loops after one more red().
+
+
+def fmod_mul(a, b, p):
+    pf = float(p)
+    hi = a * b
+    lo = exact(fma(a, b, -hi), Fraction(a) * Fraction(b) - Fraction(hi))
+    q = quot(hi, p)
+    t = exact(fma(-q, pf, hi), Fraction(hi) - Fraction(q) * p)
+    return exact(t + lo, Fraction(t) + Fraction(lo))
+
+
+@pytest.mark.parametrize("limit", [1 << 32, 1 << 44, 1 << 48])
+def test_fmod_mul_and_red_exact_over_chains(limit):
+    rng = random.Random(limit + 1)
+    for p in primes_near(limit, 6, rng):
+        x = float(rng.randint(-(p // 2), p // 2))
+        y = float(rng.randint(-(p // 2), p // 2))
+        ex, ey = int(x), int(y)
+        # chains of products as the set-up runs them (squarings, products of
+        # unreduced results, a reduction now and then): congruent, exact, and
+        # never larger than 2 p in size
+        for step in range(200):
+            if step % 3 == 0:
+                x, ex = fmod_mul(x, x, p), ex * ex % p
+            elif step % 3 == 1:
+                x, ex = fmod_mul(x, y, p), ex * ey % p
+            else:
+                x, ex = red(x + x, p), 2 * ex % p
+            assert (int(x) - ex) % p == 0
+            assert abs(x) <= 2 * p
+        r = red(x, p)
+        assert (int(r) - ex) % p == 0 and abs(r) <= 0.75 * p
