@@ -369,8 +369,9 @@ bool admit(ctd_hip_server_impl *S, ServeJob *J)
     // job with many indices or a costly family is cut finer, so its tiles
     // spread over every stream instead of holding one
     const double primes_per_int = 1.0 / std::log((double)std::max<uint64_t>(q.f1, 16));
+    const double window = J->plan.f1 > J->plan.f0 ? (double)(J->plan.f1 - J->plan.f0) : 0.0;
     const double want = std::max(SERVE_TILE_WORK / ((double)N * J->plan.ops * primes_per_int), // integers
-                                 (double)(J->plan.f1 - J->plan.f0) / SERVE_JOB_TILES);
+                                 window / SERVE_JOB_TILES);
     const uint64_t segs = std::max<uint64_t>(1, (uint64_t)(want / (double)SEG_SPAN));
     const uint64_t tl = std::min<uint64_t>(S->opts.tile_length, segs * SEG_SPAN);
     if (J->plan.f0 < J->plan.f1) {
