@@ -452,3 +452,108 @@ def test_fmod_mul_and_red_exact_over_chains(limit):
             assert abs(x) <= 2 * p
         r = red(x, p)
         assert (int(r) - ex) % p == 0 and abs(r) <= 0.75 * p
+
+
+# ---------------------------------------------------------------- theta form (Fibonacci / Lucas)
+#
+# The theta stride form works in Z_p[phi] / (phi^2 - phi - 1) with
+# theta = -(1 + phi): F(n) = 0 mod p <=> theta^n = 1 and L(n) = 0 mod p <=>
+# theta^n = -1 for p != 2, 5; the targets eps theta^-j have v coordinates
+# v_(j+1) = -3 v_j - v_(j-1) from v_0 = 0, v_1 = eps; every power of theta
+# has norm u^2 + u v - v^2 = 1, so a v coordinate leaves two candidates u.
+# Exact integer arithmetic mod p, independent of the kernel's FP64 forms.
+
+
+def rphi_mul(x, y, p):
+    (xu, xv), (yu, yv) = x, y
+    a, b = xu * yu, xv * yv
+    c = (xu + xv) * (yu + yv)
+    return ((a + b) % p, (c - a) % p)
+
+
+def rphi_pow(x, e, p):
+    r = (1, 0)
+    while e:
+        if e & 1:
+            r = rphi_mul(r, x, p)
+        x = rphi_mul(x, x, p)
+        e >>= 1
+    return r
+
+
+def fib_lucas(n, p):
+    """(F(n), L(n)) mod p by fast doubling."""
+    def fd(k):
+        if k == 0:
+            return (0, 1)
+        a, b = fd(k >> 1)
+        c = a * ((2 * b - a) % p) % p
+        d = (a * a + b * b) % p
+        return (d, (c + d) % p) if k & 1 else (c, d)
+    f, f1 = fd(n)
+    return f, (2 * f1 - f) % p
+
+
+def test_theta_form_algebra():
+    import sympy
+
+    rng = random.Random(5)
+    theta = lambda p: ((-1) % p, (-1) % p)
+    primes = [3, 7, 11, 13, 29, 89, 1009] + [int(sympy.nextprime(rng.randrange(1 << 30, 1 << 47))) for _ in range(12)]
+    for p in primes:
+        t = theta(p)
+        assert rphi_mul(t, t, p) == rphi_mul(t, t, p)
+        # the product formula against the definition with phi^2 = phi + 1
+        for _ in range(10):
+            x = (rng.randrange(p), rng.randrange(p))
+            y = (rng.randrange(p), rng.randrange(p))
+            want = ((x[0] * y[0] + x[1] * y[1]) % p, (x[0] * y[1] + x[1] * y[0] + x[1] * y[1]) % p)
+            assert rphi_mul(x, y, p) == want
+        # hits: F(n) = 0 <=> theta^n = 1, L(n) = 0 <=> theta^n = -1, and norm 1
+        ns = list(range(0, 200)) + [rng.randrange(1, 1 << 40) for _ in range(30)]
+        if p < 2000:
+            # the rank of apparition and its multiples, where hits are
+            a = next(n for n in range(1, 4 * p) if fib_lucas(n, p)[0] == 0)
+            ns += [a, 2 * a, a // 2, 3 * a, a + 1]
+        for n in ns:
+            x = rphi_pow(t, n, p)
+            f, l = fib_lucas(n, p)
+            assert (f == 0) == (x == (1, 0)), (p, n)
+            assert (l == 0) == (x == (p - 1, 0)), (p, n)
+            u, v = x
+            assert (u * u + u * v - v * v) % p == 1
+        # targets: the v coordinates of eps theta^-j follow the recurrence
+        tinv = rphi_pow(t, p * p - 2, p)  # theta^-1 (theta^(p^2-1) = 1 in the ring's unit group)
+        assert rphi_mul(t, tinv, p) == (1, 0)
+        for eps in (1, p - 1):
+            va, vb = 0, eps % p
+            y = (eps % p, 0)
+            for j in range(1, 130):
+                y = rphi_mul(y, tinv, p)
+                assert y[1] == vb, (p, eps, j)
+                va, vb = vb, (-3 * vb - va) % p
+        # a v coordinate leaves two candidates for u: u and -v - u
+        for _ in range(20):
+            n = rng.randrange(1, 1 << 30)
+            u, v = rphi_pow(t, n, p)
+            cands = [w for w in ((u, v), ((-v - u) % p, v)) if (w[0] * w[0] + w[0] * v - v * v) % p == 1]
+            assert (u, v) in cands
+
+
+def test_theta_hits_at_known_large_factors():
+    # q | F(m) for primes q in [2^37, 2^47) (tests/golden/fib_large_factors.json):
+    # theta^m = 1 in Z_q[phi], and theta^(m/2) = -1 when L(m/2) = 0 as well
+    import json
+    import os
+
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "fib_large_factors.json")
+    with open(path) as f:
+        pairs = [tuple(x) for x in json.load(f)["pairs"]]
+    assert pairs
+    for m, q in pairs:
+        t = ((-1) % q, (-1) % q)
+        assert fib_lucas(m, q)[0] == 0
+        assert rphi_pow(t, m, q) == (1, 0)
+        assert rphi_pow(t, m + 1, q) != (1, 0)
+        if m % 2 == 0 and fib_lucas(m // 2, q)[1] == 0:
+            assert rphi_pow(t, m // 2, q) == (q - 1, 0)
