@@ -512,24 +512,28 @@ def run_mixed(args, torch, ctd, nat):
     del bufs
     torch.cuda.empty_cache()
 
-    # short jobs: the same distribution on 2^24-integer windows, where one job
-    # no longer fills the device
-    short = None
-    if args.short_jobs and not args.no_plain:
-        sj = workloads.mixed_jobs(args.short_jobs, window_bits=24)
+    # short jobs: the same distribution on 2^24- and 2^20-integer windows,
+    # where one job no longer fills the device
+    def window_block(bits):
+        sj = workloads.mixed_jobs(args.short_jobs, window_bits=bits)
         sc = [nat.CtdJob(j.seq_id, j.max_factors, j.param, j.n0, j.n1, j.f0, j.f1) for j in sj]
         # warm-up, as for the main batch: the launch shapes of every (family,
         # form, index class) of these jobs are tuned before the timed step (on
         # the first 2^20 integers of each window)
         eng.run_batch([nat.CtdJob(j.seq_id, j.max_factors, j.param, j.n0, j.n1, j.f0, j.f0 + (1 << 20)) for j in sj])
         sb, sbufs, (sp, _) = batched_vs_plain(torch, eng, nat, sc, mx, flags, tile, args.batch_streams, 1,
-                                              label="2^24-window mix")
+                                              label=f"2^{bits}-window mix")
         del sbufs
         torch.cuda.empty_cache()
-        short = {"workload": f"{len(sc)} jobs of the mixed-job distribution with 2^24-wide prime windows",
-                 "batched": {k: sb[k] for k in ("value", "seconds", "updates", "tiles", "graphs", "shapes_tuned")},
-                 "plain": {k: sp[k] for k in ("value", "seconds", "updates", "factors_identical_to_batched")},
-                 "batched_vs_plain": sb["batched_vs_plain"]}
+        return {"workload": f"{len(sc)} jobs of the mixed-job distribution with 2^{bits}-wide prime windows",
+                "batched": {k: sb[k] for k in ("value", "seconds", "updates", "tiles", "graphs", "shapes_tuned")},
+                "plain": {k: sp[k] for k in ("value", "seconds", "updates", "factors_identical_to_batched")},
+                "batched_vs_plain": sb["batched_vs_plain"]}
+
+    short = tiny = None
+    if args.short_jobs and not args.no_plain:
+        short = window_block(24)
+        tiny = window_block(20)
 
     serving = serve_block(args, eng, ctd, nat, jobs, cjobs, batched_digests) if args.serve_block_jobs else None
 
@@ -614,6 +618,7 @@ def run_mixed(args, torch, ctd, nat):
         "plain": result_plain,
         "batched_vs_plain": (value / result_plain["value"]) if result_plain else None,
         "short_windows": short,
+        "tiny_windows": tiny,
         "sieve": sieve,
         "serving": serving,
         "roofline": roof,
