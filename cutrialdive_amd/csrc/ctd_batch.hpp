@@ -110,10 +110,12 @@ struct Tile {
 };
 
 // Lanes a tile's residue launch should offer (the index range is split into
-// chunks until it does): the batched graph keeps its streams full with 2048
-// per CU (8 waves), and fewer chunks mean less per-chunk set-up; the serving
+// chunks until it does): the batched graph's concurrent tiles keep the device
+// full with 512 per CU, and fewer chunks mean less per-chunk set-up (2000
+// jobs of 2^20-integer windows: 6.90e12 updates/s against 6.47e12 with 2048;
+// tiles of 2^24 integers and more hold enough primes either way); the serving
 // form, whose streams run dry between scheduler passes, measured best at 4096.
-constexpr uint32_t BATCH_LANES_PER_CU = 2048;
+constexpr uint32_t BATCH_LANES_PER_CU = 512;
 constexpr uint32_t SERVE_LANES_PER_CU = 4096;
 
 // VALU instructions per update of the family's loop (FP64 forms below 2^48,
